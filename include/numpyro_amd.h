@@ -1,0 +1,186 @@
+/* numpyro_amd: MI355X-native HMC/NUTS engine, C ABI.
+ *
+ * Plain pointers and sizes only.  Every pointer named `d_*` is a device
+ * pointer on the current HIP device; all other pointers are host memory.
+ * Every call returns 0 on success or a negative NAX_E* code; the message is
+ * available from nax_last_error().  Numerical events (NaN energies,
+ * divergences) are never errors: they become divergence flags, as in
+ * numpyro/infer/hmc_util.py:871-874.
+ *
+ * Each entry point cites the reference interface it replaces (numpyro
+ * 0.21.0, paths relative to the reference repository root).
+ */
+#ifndef NUMPYRO_AMD_H
+#define NUMPYRO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NAX_OK 0
+#define NAX_EINVAL (-22)
+#define NAX_ENOMEM (-12)
+#define NAX_EHIP (-5)
+#define NAX_ESTATE (-71)
+
+#define NAX_ALGO_NUTS 0
+#define NAX_ALGO_HMC 1
+
+#define NAX_X_F32 0
+#define NAX_X_BF16 1
+#define NAX_X_FP8 2
+
+typedef struct nax_sampler nax_sampler;
+
+/* Kernel configuration: the keyword arguments of hmc()'s init_kernel
+ * (numpyro/infer/hmc.py:193-212) and of NUTS/HMC (hmc.py:625-686, 916-951).
+ * A non-positive trajectory_length / num_steps means None. */
+typedef struct {
+  int32_t num_chains;
+  int32_t dim;
+  int32_t algo;
+  int32_t num_warmup;
+  int32_t max_tree_depth_warmup;
+  int32_t max_tree_depth;
+  int32_t adapt_step_size;
+  int32_t adapt_mass_matrix;
+  int32_t regularize_mass_matrix;
+  int32_t partitionable_keys; /* jax_threefry_partitionable (jax>=0.5 default: 1) */
+  int32_t num_steps;
+  int32_t reserved;
+  double step_size;
+  double target_accept_prob;
+  double trajectory_length;
+  double max_delta_energy; /* hmc.py:188: 1000 */
+} nax_config;
+
+/* Per-iteration collection buffers, fori_collect semantics
+ * (numpyro/util.py:321-454).  Any pointer may be NULL (not collected).
+ * Shapes: z [C, size, dim] fp32; the others [C, size]. */
+typedef struct {
+  int64_t start_idx;
+  int32_t thinning;
+  int32_t size;
+  float* d_z;
+  double* d_potential_energy;
+  double* d_energy;
+  float* d_accept_prob;
+  float* d_mean_accept_prob;
+  int32_t* d_num_steps;
+  uint8_t* d_diverging;
+  float* d_step_size;
+} nax_collect;
+
+/* Host view of the per-chain state (HMCState, hmc.py:31-48, with the
+ * HMCAdaptState of hmc_util.py:16-30).  Device arrays are [C, dim] fp32
+ * unless stated; host scalars are [C]. */
+typedef struct {
+  float* d_z;
+  float* d_z_grad;
+  float* d_inverse_mass_matrix;
+  float* d_mass_matrix_sqrt;
+  int32_t* i;
+  double* potential_energy;
+  double* energy;
+  double* accept_prob;
+  double* mean_accept_prob;
+  int32_t* num_steps;
+  int32_t* diverging;
+  double* step_size;
+  uint32_t* rng_key; /* [C, 2] */
+} nax_state_view;
+
+const char* nax_last_error(void);
+
+/* Replaces hmc(potential_fn, algo=...) (hmc.py:113) + init_kernel's static
+ * configuration (hmc.py:193-212). */
+int nax_create(const nax_config* cfg, nax_sampler** out);
+void nax_destroy(nax_sampler* h);
+
+/* Built-in potential: Bayesian logistic regression, the model of
+ * examples/covtype.py:66-71 (coefs ~ Normal(0, prior_scale), obs ~
+ * Bernoulli(logits=X @ coefs)), replacing potential_energy
+ * (numpyro/infer/util.py:333-358) + jax.value_and_grad (hmc_util.py:242-252).
+ * d_x: [n, d] row-major fp32; d_y: [n] fp32 in {0,1}.  x_dtype selects the
+ * storage the kernels read (fp32, bf16 or fp8 e4m3 with per-column scales);
+ * the library keeps its own converted copy for bf16/fp8. */
+int nax_model_logistic(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
+                       int32_t d, double prior_scale, int32_t x_dtype);
+
+/* Potential supplied by the caller (a potential_fn, hmc.py:127-130): the
+ * engine asks for U and dU at d_z_request, the caller writes them into the
+ * gradient buffers, see nax_advance. */
+int nax_model_external(nax_sampler* h);
+
+/* Initial state: init_kernel (hmc.py:193-362) for C chains.
+ * d_z0 [C, dim]; keys [C, 2] host (the per-chain rng_key of init_kernel);
+ * d_inverse_mass_matrix [dim] or NULL (identity). */
+int nax_init(nax_sampler* h, const float* d_z0, const uint32_t* keys,
+             const float* d_inverse_mass_matrix);
+
+/* Runs sample_kernel (hmc.py:459-530) on every chain until each has done
+ * `target_iter` iterations in total, collecting per `col` (may be NULL).
+ * Chains run their iterations asynchronously between gradient evaluations.
+ * Built-in potentials only.  *grad_evals (may be NULL) receives the number of
+ * useful gradient evaluations summed over chains. */
+int nax_run(nax_sampler* h, int64_t target_iter, const nax_collect* col, int64_t* grad_evals);
+
+/* One state-machine step for an external potential: consumes the
+ * gradients in the buffers of nax_buffers(), advances each chain to its next
+ * gradient request (or to target_iter), and returns in *pending the number
+ * of chains that requested a gradient (0: done). */
+int nax_advance(nax_sampler* h, int64_t target_iter, const nax_collect* col, int32_t* pending);
+
+/* Device buffers of the gradient exchange: z requests [C, ld] fp32,
+ * potential energies [C] fp64, gradients [C, ld] fp32. */
+int nax_buffers(nax_sampler* h, float** d_z_request, double** d_potential_energy,
+                float** d_grad, int32_t* ld);
+
+/* Copies the requested positions out to d_z_out [C, dim] and/or the
+ * caller's U [C] fp64 and dU [C, dim] fp32 in (any pointer may be NULL);
+ * the dense-layout form of nax_buffers. */
+int nax_exchange(nax_sampler* h, float* d_z_out, const double* d_pe_in, const float* d_grad_in);
+
+/* HMCState / post_warmup_state access (mcmc.py:558-587): copies the
+ * per-chain state out of / into the sampler. */
+int nax_get_state(nax_sampler* h, const nax_state_view* v);
+int nax_set_state(nax_sampler* h, const nax_state_view* v);
+
+/* Potential and gradient of the built-in model at d_z [count, ld]:
+ * potential_energy + value_and_grad (util.py:333-358, hmc_util.py:242-252). */
+int nax_potential_grad(nax_sampler* h, const float* d_z, int32_t ld, int32_t count,
+                       double* d_pe, float* d_grad);
+
+/* L velocity_verlet steps (hmc_util.py:262-311) for C chains in place:
+ * d_z, d_r, d_grad [C, dim]; d_pe [C]; d_inverse_mass_matrix [dim]. */
+int nax_leapfrog(nax_sampler* h, float* d_z, float* d_r, float* d_grad, double* d_pe,
+                 double step_size, int32_t num_steps, const float* d_inverse_mass_matrix);
+
+/* jax.random primitives (the call sites listed in hmc.py:102, 335, 472 and
+ * hmc_util.py:804, 1162): threefry2x32-20 over counters; random.split;
+ * random.normal(key, (n,)). keys/counters are device arrays. */
+int nax_threefry2x32(const uint32_t* d_key /*2*/, const uint32_t* d_x0, const uint32_t* d_x1,
+                     uint32_t* d_out0, uint32_t* d_out1, int64_t n);
+int nax_random_split(const uint32_t* d_key /*2*/, int32_t num, int32_t partitionable,
+                     uint32_t* d_out /*[num,2]*/);
+int nax_random_normal(const uint32_t* d_key /*2*/, int32_t n, int32_t partitionable,
+                      float* d_out);
+
+/* Per-chain init_kernel keys the way MCMC.run + HMC.init derive them from
+ * one key: split(key, C)[c] (mcmc.py:670-671), then split(.)[0]
+ * (hmc.py:744-750).  d_out [C, 2]. */
+int nax_chain_keys(const uint32_t* d_key /*2*/, int32_t num_chains, int32_t partitionable,
+                   uint32_t* d_out);
+
+/* Timing of the dominant (gradient) kernel inside nax_run, measured with HIP
+ * events on the sampler's stream: total milliseconds and launches since the
+ * last reset. */
+int nax_kernel_timing(nax_sampler* h, int32_t enable, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,251 @@
+// Bayesian logistic regression potential and its hand-written adjoint for C
+// chains at once.
+//
+//   U(b) = sum_n [softplus(l_n) - y_n l_n] + sum_d [b_d^2 / (2 s^2) + log(sqrt(2 pi) s)]
+//   dU   = X^T (sigmoid(l) - y) + b / s^2,        l = X b
+//
+// which is -(BernoulliLogits.log_prob + Normal.log_prob) of the reference
+// (numpyro/distributions/discrete.py:244-263, distributions/util.py:317-320,
+// distributions/continuous.py:2975-2989) for the model of
+// examples/covtype.py:66-71, with jax.value_and_grad (hmc_util.py:242-252)
+// replaced by the analytic adjoint.  Over C chains, L = X Z^T and
+// G = X^T R are two GEMMs that share each X row tile: a block loads a tile
+// of X once and uses it for both.  Row tiles are split over blocks; each
+// block writes fp32 gradient partials and fp64 energy partials, and a second
+// kernel sums the partials in a fixed order (deterministic) and adds the
+// prior.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "nax_device.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+__device__ __forceinline__ float softplusf_(float x) {
+  // max(x, 0) + log1p(exp(-|x|))  (distributions/util.py:317-320)
+  return fmaxf(x, 0.0f) + log1pf(__expf(-fabsf(x)));
+}
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// ---- plain FP32 form ----------------------------------------------------------
+// Block: 256 threads, BN rows x BC chains per step.  DCH = ceil(D/64) column
+// chunks held per thread in the gradient accumulator.
+template <int BN, int BC, int DCH>
+__global__ void __launch_bounds__(256)
+logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int64_t N, int D,
+                  const float* __restrict__ Z, int ldz, int C, int RB,
+                  float* __restrict__ wsg, double* __restrict__ wsu) {
+  extern __shared__ float smem[];
+  const int DP = DCH * 64 + 1;
+  float* Zs = smem;                       // [BC][DP]
+  float* Xs = Zs + BC * DP;               // [BN][DP]
+  float* Rs = Xs + BN * DP;               // [BN][BC]
+  double* Ud = (double*)(Rs + BN * BC);   // [256]
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.y * BC;
+  const int rb = blockIdx.x;
+
+  for (int i = t; i < BC * DP; i += 256) {
+    int c = i / DP, d = i % DP;
+    Zs[i] = (c0 + c < C && d < D) ? Z[(size_t)(c0 + c) * ldz + d] : 0.0f;
+  }
+  // gradient accumulators: thread owns d = (t & 63) + 64 j, c = (t >> 6) + 4 k
+  constexpr int CK = BC / 4;
+  float acc[DCH][CK];
+#pragma unroll
+  for (int j = 0; j < DCH; ++j)
+#pragma unroll
+    for (int k = 0; k < CK; ++k) acc[j][k] = 0.0f;
+  double uacc = 0.0;  // per (row, chain) pair owned in the L phase
+  constexpr int PAIRS = BN * BC / 256;
+  static_assert(PAIRS >= 1 && (BN * BC) % 256 == 0, "tile");
+
+  const int64_t ntiles = (N + BN - 1) / BN;
+  for (int64_t tile = rb; tile < ntiles; tile += RB) {
+    const int64_t n0 = tile * BN;
+    __syncthreads();
+    // X tile: rows n0..n0+BN-1 are contiguous in row-major X
+    for (int i = t; i < BN * D; i += 256) {
+      int r = i / D, d = i % D;
+      int64_t n = n0 + r;
+      Xs[r * DP + d] = (n < N) ? X[n * D + d] : 0.0f;
+    }
+    __syncthreads();
+    // L = X_tile Z^T, R = sigmoid(L) - y, U += softplus(L) - y L
+#pragma unroll
+    for (int p = 0; p < PAIRS; ++p) {
+      int q = t + 256 * p;
+      int r = q % BN, c = q / BN;
+      const float* xr = Xs + r * DP;
+      const float* zc = Zs + c * DP;
+      float l = 0.0f;
+      for (int d = 0; d < D; ++d) l = fmaf(xr[d], zc[d], l);
+      int64_t n = n0 + r;
+      float rv = 0.0f;
+      if (n < N && c0 + c < C) {
+        float yv = y[n];
+        rv = sigmoidf_(l) - yv;
+        uacc += (double)(softplusf_(l) - yv * l);
+      }
+      Rs[r * BC + c] = rv;
+    }
+    __syncthreads();
+    // G[c][d] += sum_r X[r][d] R[r][c]
+    for (int r = 0; r < BN; ++r) {
+      const float* xr = Xs + r * DP;
+      const float* rr = Rs + r * BC;
+#pragma unroll
+      for (int j = 0; j < DCH; ++j) {
+        float xv = xr[(t & 63) + 64 * j];
+#pragma unroll
+        for (int k = 0; k < CK; ++k) acc[j][k] = fmaf(xv, rr[(t >> 6) + 4 * k], acc[j][k]);
+      }
+    }
+  }
+  // write gradient partials [rb][c][d]
+#pragma unroll
+  for (int j = 0; j < DCH; ++j) {
+    int d = (t & 63) + 64 * j;
+#pragma unroll
+    for (int k = 0; k < CK; ++k) {
+      int c = c0 + (t >> 6) + 4 * k;
+      if (d < D && c < C) wsg[((size_t)rb * C + c) * D + d] = acc[j][k];
+    }
+  }
+  // energy partials: pairs q = t + 256 p have chain q / BN; with PAIRS == 1
+  // threads [c*BN, (c+1)*BN) own chain c.
+  Ud[t] = uacc;
+  __syncthreads();
+  if (t < BC && c0 + t < C) {
+    double s = 0.0;
+    for (int q = t * BN; q < (t + 1) * BN; ++q) s += Ud[q % 256];
+    wsu[(size_t)rb * C + c0 + t] = s;
+  }
+}
+
+// Sum partials in rb order, add the Normal(0, s) prior.
+__global__ void __launch_bounds__(256)
+logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB, int C,
+             int D, const float* __restrict__ Z, int ldz, double inv_s2, double log_norm,
+             double* __restrict__ pe, float* __restrict__ g, int ldg) {
+  const int c = blockIdx.x;
+  const int t = threadIdx.x;
+  __shared__ double red[256];
+  double pr = 0.0;
+  for (int d = t; d < D; d += 256) {
+    float s = 0.0f;
+    for (int r = 0; r < RB; ++r) s += wsg[((size_t)r * C + c) * D + d];
+    const float zv = Z[(size_t)c * ldz + d];
+    g[(size_t)c * ldg + d] = s + (float)(zv * inv_s2);
+    pr += 0.5 * (double)zv * zv * inv_s2 + log_norm;
+  }
+  red[t] = pr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double u = 0.0;
+    for (int r = 0; r < RB; ++r) u += wsu[(size_t)r * C + c];
+    pe[c] = u + red[0];
+  }
+}
+
+struct LogisticPotential : Potential {
+  const float* x = nullptr;
+  const float* y = nullptr;
+  int64_t n = 0;
+  int d = 0;
+  double prior_scale = 1.0;
+  int x_dtype = NAX_X_F32;
+  float* wsg = nullptr;
+  double* wsu = nullptr;
+  size_t ws_chains = 0;
+  int rb = 0;
+
+  ~LogisticPotential() override {
+    hipFree(wsg);
+    hipFree(wsu);
+  }
+  int dim() const override { return d; }
+  const char* name() const override { return "logit_partial_f32"; }
+  double alg_bytes(int C) const override { return 4.0 * (double)n * d + 4.0 * n; }
+  double alg_flops(int C) const override { return 4.0 * (double)n * d * C; }
+
+  hipError_t ensure_ws(int C, int RB) {
+    if (ws_chains >= (size_t)C && rb >= RB) return hipSuccess;
+    hipFree(wsg);
+    hipFree(wsu);
+    wsg = nullptr;
+    wsu = nullptr;
+    hipError_t e = hipMalloc(&wsg, sizeof(float) * (size_t)RB * C * d);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&wsu, sizeof(double) * (size_t)RB * C);
+    if (e != hipSuccess) return e;
+    ws_chains = C;
+    rb = RB;
+    return hipSuccess;
+  }
+
+  hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe, float* g,
+                  int ldg) override {
+    constexpr int BN = 32, BC = 8;
+    const int ctiles = (C + BC - 1) / BC;
+    const int64_t ntiles = (n + BN - 1) / BN;
+    int RB = (int)std::min<int64_t>(ntiles, std::max<int64_t>(1, 2048 / ctiles));
+    hipError_t e = ensure_ws(C, RB);
+    if (e != hipSuccess) return e;
+    const int dch = (d + 63) / 64;
+    const int DP = dch * 64 + 1;
+    size_t shm = sizeof(float) * ((size_t)BC * DP + (size_t)BN * DP + BN * BC) + 256 * sizeof(double);
+    dim3 grid(RB, ctiles);
+#define NAX_LAUNCH(DC)                                                                    \
+  hipLaunchKernelGGL((logit_partial_f32<BN, BC, DC>), grid, dim3(256), shm, s, x, y, n, d, z, \
+                     ldz, C, RB, wsg, wsu)
+    switch (dch) {
+      case 1: NAX_LAUNCH(1); break;
+      case 2: NAX_LAUNCH(2); break;
+      case 3: NAX_LAUNCH(3); break;
+      case 4: NAX_LAUNCH(4); break;
+      case 5: NAX_LAUNCH(5); break;
+      case 6: NAX_LAUNCH(6); break;
+      case 7: NAX_LAUNCH(7); break;
+      case 8: NAX_LAUNCH(8); break;
+      default: return hipErrorInvalidValue;
+    }
+#undef NAX_LAUNCH
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const double inv_s2 = 1.0 / (prior_scale * prior_scale);
+    const double log_norm = std::log(std::sqrt(2.0 * M_PI) * prior_scale);
+    hipLaunchKernelGGL(logit_reduce, dim3(C), dim3(256), 0, s, wsg, wsu, RB, C, d, z, ldz,
+                       inv_s2, log_norm, pe, g, ldg);
+    return hipGetLastError();
+  }
+};
+
+Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
+                         int x_dtype, int fuse_variant, hipStream_t s, std::string* err) {
+  if (x_dtype != NAX_X_F32) {
+    *err = "x_dtype not supported yet";
+    return nullptr;
+  }
+  if (d > 320) {
+    *err = "the fp32 logistic kernel supports dim <= 320";
+    return nullptr;
+  }
+  auto* p = new LogisticPotential();
+  p->x = x;
+  p->y = y;
+  p->n = n;
+  p->d = d;
+  p->prior_scale = prior_scale;
+  p->x_dtype = x_dtype;
+  return p;
+}
+
+}  // namespace nax

@@ -1,0 +1,162 @@
+// Device helpers shared by the engine kernels: threefry2x32-20 and the
+// jax.random key/bit/float derivations, erfinv, wave reductions.
+//
+// Random streams follow jax.random as numpyro 0.21.0 consumes it:
+//   split:   numpyro/infer/hmc.py:335,472  numpyro/infer/hmc_util.py:565,656,920,1005,1161
+//   normal:  numpyro/infer/hmc.py:102
+//   bernoulli: numpyro/infer/hmc_util.py:804,1162
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NAX_WAVE 64
+
+namespace nax {
+
+struct Key {
+  uint32_t k0, k1;
+};
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) {
+  return (x << r) | (x >> (32 - r));
+}
+
+// threefry2x32 with 20 rounds: rotations {13,15,26,6} / {17,29,16,24},
+// key injection after every 4 rounds (Salmon et al., SC'11).
+__device__ __forceinline__ void threefry2x32(Key k, uint32_t& x0, uint32_t& x1) {
+  const uint32_t ks0 = k.k0, ks1 = k.k1, ks2 = k.k0 ^ k.k1 ^ 0x1BD11BDAu;
+  x0 += ks0;
+  x1 += ks1;
+#define NAX_R(r) { x0 += x1; x1 = rotl32(x1, r); x1 ^= x0; }
+  NAX_R(13) NAX_R(15) NAX_R(26) NAX_R(6)
+  x0 += ks1; x1 += ks2 + 1u;
+  NAX_R(17) NAX_R(29) NAX_R(16) NAX_R(24)
+  x0 += ks2; x1 += ks0 + 2u;
+  NAX_R(13) NAX_R(15) NAX_R(26) NAX_R(6)
+  x0 += ks0; x1 += ks1 + 3u;
+  NAX_R(17) NAX_R(29) NAX_R(16) NAX_R(24)
+  x0 += ks1; x1 += ks2 + 4u;
+  NAX_R(13) NAX_R(15) NAX_R(26) NAX_R(6)
+  x0 += ks2; x1 += ks0 + 5u;
+#undef NAX_R
+}
+
+// jax "threefry_2x32(key, iota(n))" word i (original derivation): the count
+// array [0..n) (padded to even length) is cut into halves that form the two
+// cipher inputs; output = concat(out0, out1).
+__device__ __forceinline__ uint32_t tf_iota_word(Key k, uint32_t i, uint32_t n) {
+  const uint32_t h = (n + 1) / 2;
+  const uint32_t j = (i < h) ? i : i - h;
+  uint32_t x0 = j, x1 = (j + h < n) ? j + h : 0u;
+  threefry2x32(k, x0, x1);
+  return (i < h) ? x0 : x1;
+}
+
+// random.split(key, num)[idx]
+__device__ __forceinline__ Key split_key(Key k, uint32_t idx, uint32_t num, bool part) {
+  if (part) {
+    uint32_t x0 = 0, x1 = idx;
+    threefry2x32(k, x0, x1);
+    return Key{x0, x1};
+  }
+  return Key{tf_iota_word(k, 2 * idx, 2 * num), tf_iota_word(k, 2 * idx + 1, 2 * num)};
+}
+
+// 32-bit random_bits(key, (n,))[i]
+__device__ __forceinline__ uint32_t bits_vec(Key k, uint32_t i, uint32_t n, bool part) {
+  if (part) {
+    uint32_t x0 = 0, x1 = i;
+    threefry2x32(k, x0, x1);
+    return x0 ^ x1;
+  }
+  return tf_iota_word(k, i, n);
+}
+
+// 32-bit random_bits(key, ()) (scalar shape)
+__device__ __forceinline__ uint32_t bits_scalar(Key k, bool part) {
+  uint32_t x0 = 0, x1 = 0;
+  threefry2x32(k, x0, x1);
+  return part ? (x0 ^ x1) : x0;
+}
+
+// [0,1) float32 from 32 random bits: (bits >> 9) | 0x3F800000, minus 1.
+__device__ __forceinline__ float bits_to_unit(uint32_t b) {
+  return __uint_as_float((b >> 9) | 0x3F800000u) - 1.0f;
+}
+
+// Inverse error function, single precision, from M. Giles, "Approximating
+// the erfinv function" (GPU Computing Gems Jade, 2011): central branch for
+// w = -log(1-x^2) < 5, tail branch in sqrt(w) otherwise.
+__device__ __forceinline__ float erfinv_giles(float x) {
+  float w = -log1pf(-x * x);
+  float p;
+  if (w < 5.0f) {
+    w = w - 2.5f;
+    p = 2.81022636e-08f;
+    p = fmaf(p, w, 3.43273939e-07f);
+    p = fmaf(p, w, -3.5233877e-06f);
+    p = fmaf(p, w, -4.39150654e-06f);
+    p = fmaf(p, w, 0.00021858087f);
+    p = fmaf(p, w, -0.00125372503f);
+    p = fmaf(p, w, -0.00417768164f);
+    p = fmaf(p, w, 0.246640727f);
+    p = fmaf(p, w, 1.50140941f);
+  } else {
+    w = sqrtf(w) - 3.0f;
+    p = -0.000200214257f;
+    p = fmaf(p, w, 0.000100950558f);
+    p = fmaf(p, w, 0.00134934322f);
+    p = fmaf(p, w, -0.00367342844f);
+    p = fmaf(p, w, 0.00573950773f);
+    p = fmaf(p, w, -0.0076224613f);
+    p = fmaf(p, w, 0.00943887047f);
+    p = fmaf(p, w, 1.00167406f);
+    p = fmaf(p, w, 2.83297682f);
+  }
+  return p * x;
+}
+
+// random.normal(key, (n,))[i]: u ~ U(nextafter(-1, 0), 1) in float32
+// (f * 2 + lo, clamped at lo), then sqrt(2) * erfinv(u).
+__device__ __forceinline__ float normal_vec(Key k, uint32_t i, uint32_t n, bool part) {
+  const float lo = -0.99999994f;  // nextafter(-1, 0) in float32
+  float f = bits_to_unit(bits_vec(k, i, n, part));
+  float u = fmaxf(lo, f * 2.0f + lo);
+  return 1.41421356237309515f * erfinv_giles(u);
+}
+
+// random.uniform(key, (n,), minval, maxval)[i]
+__device__ __forceinline__ float uniform_vec(Key k, uint32_t i, uint32_t n, bool part,
+                                             float lo, float hi) {
+  float f = bits_to_unit(bits_vec(k, i, n, part));
+  return fmaxf(lo, f * (hi - lo) + lo);
+}
+
+// random.bernoulli(key, p) for a scalar p: uniform(key, ()) < p.
+__device__ __forceinline__ bool bernoulli(Key k, double p, bool part) {
+  return (double)bits_to_unit(bits_scalar(k, part)) < p;
+}
+
+// ---- wave (64-lane) reductions ---------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NAX_WAVE);
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NAX_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double log_add_exp(double a, double b) {
+  double m = fmax(a, b);
+  if (m == -INFINITY) return -INFINITY;
+  return m + log1p(exp(-fabs(a - b)));
+}
+
+__device__ __forceinline__ double expit(double x) {
+  return x >= 0 ? 1.0 / (1.0 + exp(-x)) : exp(x) / (1.0 + exp(x));
+}
+
+}  // namespace nax

@@ -1,0 +1,85 @@
+// Host-side internals shared by the translation units of libnumpyro_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/numpyro_amd.h"
+
+namespace nax {
+
+constexpr int kMaxWindows = 64;
+
+// Engine configuration passed by value to the kernels.
+struct EngineCfg {
+  int C, D, Dp, nvec;
+  int algo;  // 0 NUTS, 1 HMC
+  int num_warmup;
+  int max_depth_warmup, max_depth_post, max_depth_alloc;
+  int adapt_step_size, adapt_mass_matrix, regularize;
+  int partitionable;
+  int fixed_num_steps;        // <= 0: None
+  double trajectory_length;   // <= 0: None
+  double target_accept, max_delta_energy;
+  int num_windows;
+  int window_end[kMaxWindows];
+};
+
+// Per-chain scalar state (one struct per chain in device memory).  All
+// fields are wave-uniform: one wavefront owns one chain.
+struct ChainState {
+  // HMCState (numpyro/infer/hmc.py:31-48)
+  int32_t i, phase;
+  uint32_t rng_key[2];
+  double pe, energy, accept_prob, mean_accept_prob;
+  int32_t num_steps, diverging;
+  // build_tree loop (numpyro/infer/hmc_util.py:1088-1180)
+  uint32_t tree_key[2], sub_key[2], top_key[2], leaf_key[2];
+  int32_t going_right, depth, max_depth_cur, turning;
+  int32_t t_diverging, num_proposals, s_num, s_turning;
+  int32_t s_diverging, hmc_step, hmc_num_steps, pad0;
+  double energy_current, weight, sum_accept, peP, energyP;
+  double s_weight, s_sum_accept, s_peP, s_energyP;
+  double pe_edge, hmc_eps;
+  // HMCAdaptState (numpyro/infer/hmc_util.py:16-30)
+  double step_size, da_x, da_xavg, da_gavg, da_prox;
+  int32_t da_t, window_idx, welford_n, pad1;
+  uint32_t adapt_key[2];
+  uint32_t trans_key[2];
+};
+
+// Collection of per-iteration fields, fori_collect semantics
+// (numpyro/util.py:321-454): the state after loop index `it` goes to slot
+// (it - start_idx) / thinning, the last of each group of `thinning` wins.
+struct Collect {
+  int64_t start_idx;
+  int32_t thinning, size;
+  float* z;
+  double* potential_energy;
+  double* energy;
+  float* accept_prob;
+  float* mean_accept_prob;
+  int32_t* num_steps;
+  uint8_t* diverging;
+  float* step_size;
+};
+
+// A potential U(z) and its gradient for C chains at once.
+struct Potential {
+  virtual ~Potential() {}
+  virtual int dim() const = 0;
+  // z: [C, ldz] fp32; writes pe[C] (fp64) and g: [C, ldg] fp32.
+  virtual hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe,
+                          float* g, int ldg) = 0;
+  // Bytes / flops one eval of C chains moves / does (algorithmic figures).
+  virtual double alg_bytes(int C) const { return 0; }
+  virtual double alg_flops(int C) const { return 0; }
+  virtual const char* name() const { return "potential"; }
+};
+
+Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
+                         int x_dtype, int fuse_variant, hipStream_t s, std::string* err);
+
+void set_error(const std::string& msg);
+
+}  // namespace nax

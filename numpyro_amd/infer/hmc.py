@@ -1,0 +1,479 @@
+"""HMC / NUTS kernels with the reference's kernel API, running on the engine.
+
+Mirrors ``numpyro/infer/hmc.py`` (numpyro 0.21.0): ``hmc()`` returns
+``(init_kernel, sample_kernel)`` with the same keyword arguments and
+defaults (``hmc.py:113-538``); ``HMC`` and ``NUTS`` implement ``MCMCKernel``
+(``hmc.py:541-951``).  States are ``HMCState`` namedtuples whose leaves are
+torch tensors with a leading chain dimension.  All numerical work (leapfrog,
+tree building, adaptation, RNG, potential and gradient) runs in the HIP
+library; this module only marshals arguments.
+"""
+from collections import namedtuple
+import ctypes as C
+import math
+import warnings
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .. import random as nrandom
+from ..models import Model, PotentialFn
+from .mcmc import MCMCKernel
+
+HMCState = namedtuple(
+    "HMCState",
+    ["i", "z", "z_grad", "potential_energy", "energy", "r", "trajectory_length",
+     "num_steps", "accept_prob", "mean_accept_prob", "diverging", "adapt_state",
+     "rng_key"])
+HMCAdaptState = namedtuple(
+    "HMCAdaptState",
+    ["step_size", "inverse_mass_matrix", "mass_matrix_sqrt", "mass_matrix_sqrt_inv",
+     "ss_state", "mm_state", "window_idx", "rng_key"])
+
+FIELD_DTYPES = {
+    "potential_energy": torch.float64, "energy": torch.float64,
+    "accept_prob": torch.float32, "mean_accept_prob": torch.float32,
+    "num_steps": torch.int32, "diverging": torch.uint8,
+    "adapt_state.step_size": torch.float32,
+}
+_COLLECT_PTR = {
+    "potential_energy": "d_potential_energy", "energy": "d_energy",
+    "accept_prob": "d_accept_prob", "mean_accept_prob": "d_mean_accept_prob",
+    "num_steps": "d_num_steps", "diverging": "d_diverging",
+    "adapt_state.step_size": "d_step_size",
+}
+
+
+class Engine:
+    """One sampler handle: C chains of one potential on the current GPU."""
+
+    def __init__(self, model, num_chains, algo, num_warmup, step_size=1.0,
+                 adapt_step_size=True, adapt_mass_matrix=True, dense_mass=False,
+                 target_accept_prob=0.8, num_steps=None, trajectory_length=2 * math.pi,
+                 max_tree_depth=10, find_heuristic_step_size=False,
+                 regularize_mass_matrix=True, partitionable=None):
+        if algo not in ("HMC", "NUTS"):
+            raise ValueError("`algo` must be one of `HMC` or `NUTS`.")
+        if dense_mass not in (False, [], None):
+            raise ValueError("dense_mass is not supported by this engine (diagonal mass only)")
+        if find_heuristic_step_size:
+            raise ValueError("find_heuristic_step_size is not supported by this engine")
+        if not torch.cuda.is_available():
+            raise RuntimeError("numpyro_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.model = model
+        self.C = int(num_chains)
+        self.D = int(model.dim)
+        self.algo = algo
+        self.num_warmup = int(num_warmup)
+        mtd = max_tree_depth if isinstance(max_tree_depth, tuple) else (max_tree_depth,) * 2
+        cfg = _lib.nax_config()
+        cfg.num_chains = self.C
+        cfg.dim = self.D
+        cfg.algo = _lib.NAX_ALGO_NUTS if algo == "NUTS" else _lib.NAX_ALGO_HMC
+        cfg.num_warmup = self.num_warmup
+        cfg.max_tree_depth_warmup = int(mtd[0])
+        cfg.max_tree_depth = int(mtd[1])
+        cfg.adapt_step_size = int(bool(adapt_step_size))
+        cfg.adapt_mass_matrix = int(bool(adapt_mass_matrix))
+        cfg.regularize_mass_matrix = int(bool(regularize_mass_matrix))
+        part = nrandom.PARTITIONABLE if partitionable is None else partitionable
+        self.partitionable = bool(part)
+        cfg.partitionable_keys = int(self.partitionable)
+        cfg.num_steps = int(num_steps) if num_steps is not None else 0
+        cfg.step_size = float(step_size)
+        cfg.target_accept_prob = float(target_accept_prob)
+        cfg.trajectory_length = float(trajectory_length) if trajectory_length is not None else 0.0
+        cfg.max_delta_energy = 1000.0
+        self.cfg = cfg
+        h = C.c_void_p()
+        _lib.check(self.lib.nax_create(C.byref(cfg), C.byref(h)), "nax_create")
+        self.h = h
+        model.attach(h)
+        self.external = isinstance(model, PotentialFn)
+        self.grad_evals = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self.lib.nax_destroy(h)
+            self.h = None
+
+    # -- external potential exchange ------------------------------------------
+    def _serve_external(self):
+        """Evaluates the torch potential at the engine's requested positions
+        and hands U, dU back (``nax_exchange``)."""
+        zt = torch.empty((self.C, self.D), dtype=torch.float32, device="cuda")
+        _lib.check(self.lib.nax_exchange(self.h, _lib.ptr(zt), None, None), "nax_exchange")
+        u, g = self.model.value_and_grad(zt)
+        u = u.to(torch.float64).contiguous()
+        g = g.to(torch.float32).contiguous()
+        torch.cuda.synchronize()
+        _lib.check(self.lib.nax_exchange(self.h, None, _lib.ptr(u), _lib.ptr(g)), "nax_exchange")
+
+    def _advance_external(self, target, col):
+        pending = C.c_int32()
+        evals = 0
+        while True:
+            _lib.check(self.lib.nax_advance(self.h, int(target), col, C.byref(pending)),
+                       "nax_advance")
+            if pending.value == 0:
+                return evals
+            evals += pending.value
+            self._serve_external()
+
+    # -- init / run -------------------------------------------------------------
+    def init(self, z0, keys, inverse_mass_matrix=None):
+        z0 = torch.as_tensor(z0, dtype=torch.float32).to("cuda").reshape(self.C, self.D)
+        z0 = z0.contiguous()
+        keys = np.ascontiguousarray(np.asarray(keys, dtype=np.uint32).reshape(self.C, 2))
+        imm = None
+        if inverse_mass_matrix is not None:
+            imm = torch.as_tensor(inverse_mass_matrix, dtype=torch.float32).to("cuda")
+            if imm.ndim == 2:
+                imm = torch.diagonal(imm)
+            imm = imm.reshape(self.D).contiguous()
+        torch.cuda.synchronize()
+        _lib.check(self.lib.nax_init(self.h, _lib.ptr(z0), keys.ctypes.data_as(C.c_void_p),
+                                     _lib.ptr(imm)), "nax_init")
+        if self.external:
+            self._serve_external()
+            self._advance_external(0, None)
+        self._z0 = z0
+
+    def run(self, target, collect_fields=(), start_idx=0, thinning=1, size=0):
+        """Advance every chain to `target` total iterations; returns the
+        collected [C, size, ...] tensors."""
+        out = {}
+        col = _lib.nax_collect()
+        col.start_idx = int(start_idx)
+        col.thinning = int(thinning)
+        col.size = int(size)
+        if size > 0:
+            for f in collect_fields:
+                if f == "z":
+                    out[f] = torch.zeros((self.C, size, self.D), dtype=torch.float32, device="cuda")
+                    col.d_z = out[f].data_ptr()
+                elif f in _COLLECT_PTR:
+                    out[f] = torch.zeros((self.C, size), dtype=FIELD_DTYPES[f], device="cuda")
+                    setattr(col, _COLLECT_PTR[f], out[f].data_ptr())
+                else:
+                    raise ValueError(f"cannot collect field {f!r}")
+        torch.cuda.synchronize()
+        if self.external:
+            self.grad_evals = self._advance_external(target, C.byref(col))
+        else:
+            ge = C.c_int64()
+            _lib.check(self.lib.nax_run(self.h, int(target), C.byref(col), C.byref(ge)),
+                       "nax_run")
+            self.grad_evals = ge.value
+        if "diverging" in out:
+            out["diverging"] = out["diverging"].bool()
+        return out
+
+    def timing(self, enable):
+        ms, n = C.c_double(), C.c_int64()
+        _lib.check(self.lib.nax_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)),
+                   "timing")
+        return ms.value, n.value
+
+    # -- state --------------------------------------------------------------------
+    def get_state(self):
+        Cn, D = self.C, self.D
+        t = {k: torch.empty((Cn, D), dtype=torch.float32, device="cuda")
+             for k in ("z", "z_grad", "imm", "msqrt")}
+        h = {"i": np.zeros(Cn, np.int32), "pe": np.zeros(Cn), "energy": np.zeros(Cn),
+             "ap": np.zeros(Cn), "map": np.zeros(Cn), "ns": np.zeros(Cn, np.int32),
+             "div": np.zeros(Cn, np.int32), "ss": np.zeros(Cn),
+             "key": np.zeros(2 * Cn, np.uint32)}
+        v = _lib.nax_state_view(
+            t["z"].data_ptr(), t["z_grad"].data_ptr(), t["imm"].data_ptr(),
+            t["msqrt"].data_ptr(), h["i"].ctypes.data, h["pe"].ctypes.data,
+            h["energy"].ctypes.data, h["ap"].ctypes.data, h["map"].ctypes.data,
+            h["ns"].ctypes.data, h["div"].ctypes.data, h["ss"].ctypes.data,
+            h["key"].ctypes.data)
+        torch.cuda.synchronize()
+        _lib.check(self.lib.nax_get_state(self.h, C.byref(v)), "nax_get_state")
+        adapt = HMCAdaptState(torch.from_numpy(h["ss"]), t["imm"], t["msqrt"],
+                              1.0 / t["msqrt"], None, None, None, None)
+        return HMCState(torch.from_numpy(h["i"]), t["z"], t["z_grad"],
+                        torch.from_numpy(h["pe"]), torch.from_numpy(h["energy"]), None,
+                        self.cfg.trajectory_length or None, torch.from_numpy(h["ns"]),
+                        torch.from_numpy(h["ap"]), torch.from_numpy(h["map"]),
+                        torch.from_numpy(h["div"]).bool(), adapt,
+                        h["key"].reshape(Cn, 2))
+
+    def set_state(self, st, rng_key=None):
+        Cn = self.C
+        z = torch.as_tensor(st.z, dtype=torch.float32).to("cuda").reshape(Cn, self.D).contiguous()
+        zg = torch.as_tensor(st.z_grad, dtype=torch.float32).to("cuda").reshape(Cn, self.D).contiguous()
+        imm = torch.as_tensor(st.adapt_state.inverse_mass_matrix, dtype=torch.float32).to("cuda")
+        msq = torch.as_tensor(st.adapt_state.mass_matrix_sqrt, dtype=torch.float32).to("cuda")
+        imm = imm.expand(Cn, self.D).contiguous()
+        msq = msq.expand(Cn, self.D).contiguous()
+        keys = np.asarray(st.rng_key if rng_key is None else rng_key, np.uint32)
+        keys = np.ascontiguousarray(np.broadcast_to(keys.reshape(-1, 2), (Cn, 2))).ravel()
+        arr = lambda x, dt: np.ascontiguousarray(np.broadcast_to(  # noqa: E731
+            np.asarray(torch.as_tensor(x).cpu().numpy(), dt), (Cn,)))
+        i = arr(st.i, np.int32)
+        pe = arr(st.potential_energy, np.float64)
+        en = arr(st.energy, np.float64)
+        ap = arr(st.accept_prob, np.float64)
+        mp = arr(st.mean_accept_prob, np.float64)
+        ns = arr(st.num_steps, np.int32)
+        dv = arr(st.diverging, np.int32)
+        ss = arr(st.adapt_state.step_size, np.float64)
+        v = _lib.nax_state_view(z.data_ptr(), zg.data_ptr(), imm.data_ptr(), msq.data_ptr(),
+                                i.ctypes.data, pe.ctypes.data, en.ctypes.data, ap.ctypes.data,
+                                mp.ctypes.data, ns.ctypes.data, dv.ctypes.data, ss.ctypes.data,
+                                keys.ctypes.data)
+        torch.cuda.synchronize()
+        _lib.check(self.lib.nax_set_state(self.h, C.byref(v)), "nax_set_state")
+
+
+def _as_model(potential_fn, init_params):
+    if isinstance(potential_fn, Model):
+        return potential_fn
+    if callable(potential_fn):
+        if init_params is None:
+            raise ValueError("Valid value of `init_params` must be provided with `potential_fn`.")
+        z = torch.as_tensor(init_params)
+        return PotentialFn(potential_fn, z.shape[-1] if z.ndim > 0 else 1)
+    raise ValueError("potential_fn must be a numpyro_amd model or a torch callable")
+
+
+def hmc(potential_fn=None, potential_fn_gen=None, kinetic_fn=None, algo="NUTS"):
+    """``numpyro/infer/hmc.py:113``: returns ``(init_kernel, sample_kernel)``.
+
+    ``potential_fn`` is a built-in model (``numpyro_amd.models``) or a torch
+    callable on one flat latent vector.  ``init_params`` carry a leading chain
+    dimension when several chains are initialised at once."""
+    if kinetic_fn is not None:
+        raise ValueError("custom kinetic_fn is not supported (Euclidean kinetic energy only)")
+    if algo not in {"HMC", "NUTS"}:
+        raise ValueError("`algo` must be one of `HMC` or `NUTS`.")
+    if potential_fn is not None and potential_fn_gen is not None:
+        raise ValueError("Only one of `potential_fn` or `potential_fn_gen` must be provided.")
+    box = {}
+
+    def init_kernel(init_params, num_warmup, step_size=1.0, inverse_mass_matrix=None,
+                    adapt_step_size=True, adapt_mass_matrix=True, dense_mass=False,
+                    target_accept_prob=0.8, *, num_steps=None, trajectory_length=2 * math.pi,
+                    max_tree_depth=10, find_heuristic_step_size=False,
+                    forward_mode_differentiation=False, regularize_mass_matrix=True,
+                    model_args=(), model_kwargs=None, rng_key=None):
+        pf = potential_fn
+        if potential_fn_gen is not None:
+            pf = potential_fn_gen(*model_args, **(model_kwargs or {}))
+        z0 = torch.as_tensor(np.asarray(init_params) if not torch.is_tensor(init_params)
+                             else init_params, dtype=torch.float32)
+        if z0.ndim == 0:
+            z0 = z0.reshape(1)
+        model = _as_model(pf, z0)
+        if z0.ndim == 1:
+            z0 = z0[None]
+        num_chains = z0.shape[0]
+        if rng_key is None:
+            rng_key = nrandom.PRNGKey(0)
+        keys = np.asarray(rng_key, dtype=np.uint32).reshape(-1, 2)
+        if keys.shape[0] != num_chains:
+            if keys.shape[0] == 1:
+                keys = np.broadcast_to(keys, (num_chains, 2))
+            else:
+                raise ValueError("rng_key must hold one key per chain")
+        eng = Engine(model, num_chains, algo, num_warmup, step_size=step_size,
+                     adapt_step_size=adapt_step_size, adapt_mass_matrix=adapt_mass_matrix,
+                     dense_mass=dense_mass, target_accept_prob=target_accept_prob,
+                     num_steps=num_steps,
+                     trajectory_length=trajectory_length if algo == "HMC" else None,
+                     max_tree_depth=max_tree_depth,
+                     find_heuristic_step_size=find_heuristic_step_size,
+                     regularize_mass_matrix=regularize_mass_matrix)
+        eng.init(z0, keys, inverse_mass_matrix)
+        box["engine"] = eng
+        st = eng.get_state()
+        box["i"] = int(st.i.max())
+        return st
+
+    def sample_kernel(hmc_state, model_args=(), model_kwargs=None):
+        eng = box["engine"]
+        i = int(torch.as_tensor(hmc_state.i).max())
+        if i != box["i"]:
+            eng.set_state(hmc_state)
+        eng.run(i + 1)
+        st = eng.get_state()
+        box["i"] = i + 1
+        return st
+
+    init_kernel.engine = lambda: box.get("engine")
+    sample_kernel.engine = lambda: box.get("engine")
+    sample_kernel._nax = True
+    return init_kernel, sample_kernel
+
+
+class HMC(MCMCKernel):
+    """``numpyro/infer/hmc.py:541-822``."""
+
+    def __init__(self, model=None, potential_fn=None, kinetic_fn=None, step_size=1.0,
+                 inverse_mass_matrix=None, adapt_step_size=True, adapt_mass_matrix=True,
+                 dense_mass=False, target_accept_prob=0.8, num_steps=None,
+                 trajectory_length=2 * math.pi, init_strategy=None,
+                 find_heuristic_step_size=False, forward_mode_differentiation=False,
+                 regularize_mass_matrix=True):
+        if not (model is None) ^ (potential_fn is None):
+            raise ValueError("Only one of `model` or `potential_fn` must be specified.")
+        if type(self) is HMC:
+            if num_steps is None and trajectory_length is None:
+                raise ValueError(
+                    "At least one of `num_steps` or `trajectory_length` must be specified.")
+            if adapt_step_size and num_steps is not None and trajectory_length is not None:
+                warnings.warn("If both `num_steps` and `trajectory_length` are specified step "
+                              "size can't be adapted", stacklevel=2)
+        if kinetic_fn is not None:
+            raise ValueError("custom kinetic_fn is not supported (Euclidean kinetic energy only)")
+        self._model = model
+        self._potential_fn = potential_fn
+        self._num_steps = num_steps
+        self._step_size = float(step_size)
+        self._inverse_mass_matrix = inverse_mass_matrix
+        self._adapt_step_size = adapt_step_size
+        self._adapt_mass_matrix = adapt_mass_matrix
+        self._dense_mass = dense_mass
+        self._target_accept_prob = target_accept_prob
+        self._trajectory_length = trajectory_length
+        self._algo = "HMC"
+        self._max_tree_depth = 10
+        self._find_heuristic_step_size = find_heuristic_step_size
+        self._regularize_mass_matrix = regularize_mass_matrix
+        self._engine = None
+        self._postprocess_fn = None
+
+    @property
+    def model(self):
+        return self._model
+
+    @property
+    def sample_field(self):
+        return "z"
+
+    @property
+    def default_fields(self):
+        return ("z", "diverging")
+
+    def get_diagnostics_str(self, state):
+        return "{} steps of size {:.2e}. acc. prob={:.2f}".format(
+            int(torch.as_tensor(state.num_steps).max()),
+            float(torch.as_tensor(state.adapt_state.step_size).mean()),
+            float(torch.as_tensor(state.mean_accept_prob).mean()))
+
+    def _target_model(self):
+        return self._model if self._model is not None else self._potential_fn
+
+    def init(self, rng_key, num_warmup, init_params=None, model_args=(), model_kwargs={}):
+        """``hmc.py:740-800``.  ``rng_key`` is one key or [C, 2] chain keys."""
+        target = self._target_model()
+        keys = np.asarray(rng_key, dtype=np.uint32).reshape(-1, 2)
+        num_chains = keys.shape[0]
+        if init_params is None:
+            if not isinstance(target, Model):
+                raise ValueError(
+                    "Valid value of `init_params` must be provided with `potential_fn`.")
+            init_params = init_to_uniform(keys, target.dim, self_split=True)
+        z0 = torch.as_tensor(init_params, dtype=torch.float32)
+        if z0.ndim <= 1 and num_chains == 1:
+            z0 = z0.reshape(1, -1)
+        model = _as_model(target, z0[0] if z0.ndim > 1 else z0)
+        init_keys = np.stack([nrandom.split(k)[0] for k in keys]) if num_chains <= 8 else \
+            _split_first(keys)
+        eng = Engine(model, num_chains, self._algo, num_warmup, step_size=self._step_size,
+                     adapt_step_size=self._adapt_step_size,
+                     adapt_mass_matrix=self._adapt_mass_matrix, dense_mass=self._dense_mass,
+                     target_accept_prob=self._target_accept_prob, num_steps=self._num_steps,
+                     trajectory_length=self._trajectory_length if self._algo == "HMC" else None,
+                     max_tree_depth=self._max_tree_depth,
+                     find_heuristic_step_size=self._find_heuristic_step_size,
+                     regularize_mass_matrix=self._regularize_mass_matrix)
+        eng.init(z0.reshape(num_chains, -1), init_keys, self._inverse_mass_matrix)
+        self._engine = eng
+        if isinstance(model, Model) and not isinstance(model, PotentialFn):
+            self._postprocess_fn = model.postprocess
+        return eng.get_state()
+
+    def postprocess_fn(self, args, kwargs):
+        if self._postprocess_fn is None:
+            return lambda x: x
+        return self._postprocess_fn
+
+    def sample(self, state, model_args, model_kwargs):
+        """One iteration of every chain (``hmc.py:806-816``)."""
+        eng = self._engine
+        i = int(torch.as_tensor(state.i).max())
+        cur = eng.get_state()
+        if int(cur.i.max()) != i:
+            eng.set_state(state)
+        eng.run(i + 1)
+        return eng.get_state()
+
+    def __getstate__(self):
+        st = self.__dict__.copy()
+        st["_engine"] = None
+        return st
+
+
+class NUTS(HMC):
+    """``numpyro/infer/hmc.py:825-951``."""
+
+    def __init__(self, model=None, potential_fn=None, kinetic_fn=None, step_size=1.0,
+                 inverse_mass_matrix=None, adapt_step_size=True, adapt_mass_matrix=True,
+                 dense_mass=False, target_accept_prob=0.8, trajectory_length=None,
+                 max_tree_depth=10, init_strategy=None, find_heuristic_step_size=False,
+                 forward_mode_differentiation=False, regularize_mass_matrix=True):
+        super().__init__(model=model, potential_fn=potential_fn, kinetic_fn=kinetic_fn,
+                         step_size=step_size, inverse_mass_matrix=inverse_mass_matrix,
+                         adapt_step_size=adapt_step_size, adapt_mass_matrix=adapt_mass_matrix,
+                         dense_mass=dense_mass, target_accept_prob=target_accept_prob,
+                         trajectory_length=trajectory_length,
+                         find_heuristic_step_size=find_heuristic_step_size,
+                         regularize_mass_matrix=regularize_mass_matrix)
+        self._max_tree_depth = max_tree_depth
+        self._algo = "NUTS"
+
+
+def _split_first(keys):
+    """split(k)[0] for each key, on the device."""
+    out = np.empty_like(keys)
+    for c in range(keys.shape[0]):
+        out[c] = nrandom.split(keys[c])[0]
+    return out
+
+
+def init_to_uniform(keys, dim, radius=2.0, self_split=False):
+    """Uniform(-radius, radius) initial points in unconstrained space
+    (``numpyro/infer/initialization.py:95-129``), one per chain key.  The
+    key is split as in ``HMC.init`` (``hmc.py:744-750``) and the second half
+    drives ``random.uniform``; the per-site key derivation of numpyro's
+    effect handlers is not reproduced."""
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    rows = []
+    for k in keys:
+        kk = nrandom.split(k)[1] if self_split else k
+        u = _uniform(kk, dim)
+        rows.append(u * (2 * radius) - radius)
+    return torch.stack(rows)
+
+
+def _uniform(k, n):
+    # uniform [0,1) float32 from the device normal path's bits: u = (x + 1) / 2
+    # is not bit-identical to random.uniform; derive from threefry bits instead.
+    lib = _lib.load()
+    dk = nrandom._dev_u32(k)
+    x0 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    x1 = torch.arange(n, dtype=torch.int32, device="cuda")
+    o0 = torch.empty_like(x0)
+    o1 = torch.empty_like(x0)
+    _lib.check(lib.nax_threefry2x32(_lib.ptr(dk), _lib.ptr(x0), _lib.ptr(x1), _lib.ptr(o0),
+                                    _lib.ptr(o1), n), "threefry")
+    bits = (o0 ^ o1)
+    b = ((bits >> 9) & 0x7FFFFF) | 0x3F800000
+    return b.view(torch.float32) - 1.0

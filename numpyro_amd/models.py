@@ -1,0 +1,114 @@
+"""Built-in potentials, evaluated by the engine's HIP kernels.
+
+A model object stands where the reference passes ``potential_fn`` (the
+negative log joint in unconstrained space, ``numpyro/infer/util.py:333-358``)
+and ``postprocess_fn`` (constrain, ``infer/util.py:663-835``).  Latents are a
+flat vector in sorted site-name order, as ``ravel_pytree`` flattens a dict.
+"""
+import math
+
+import torch
+
+from . import _lib
+
+
+class Model:
+    """Base class: a potential U(z) over a flat latent of size ``dim``."""
+
+    dim = 0
+    site_sizes = ()  # ((name, size, transform), ...) in sorted name order
+
+    def attach(self, handle):
+        raise NotImplementedError
+
+    def postprocess(self, z):
+        """Constrained values per site for unconstrained z [..., dim]."""
+        out, o = {}, 0
+        for name, size, tf in self.site_sizes:
+            v = z[..., o:o + size]
+            if size == 1:
+                v = v[..., 0]
+            out[name] = torch.exp(v) if tf == "exp" else v
+            o += size
+        return out
+
+    def algorithmic_bytes(self, num_chains):
+        return 0.0
+
+    def algorithmic_flops(self, num_chains):
+        return 0.0
+
+
+class LogisticRegression(Model):
+    """``coefs ~ Normal(0, prior_scale)``, ``obs ~ Bernoulli(logits=X @ coefs)``
+    (``examples/covtype.py:66-71``, ``test/infer/test_mcmc.py:104-168``).
+
+    ``X`` [N, D] and ``y`` [N] are moved to the GPU as float32.  ``x_dtype``
+    picks the storage the gradient kernels read: ``"fp32"`` (plain), ``"bf16"``
+    (MFMA) or ``"fp8"`` (MFMA, e4m3 with per-column scales)."""
+
+    def __init__(self, X, y, prior_scale=1.0, x_dtype="fp32"):
+        X = torch.as_tensor(X)
+        y = torch.as_tensor(y)
+        if X.ndim != 2 or y.ndim != 1 or y.shape[0] != X.shape[0]:
+            raise ValueError("X must be [N, D] and y [N]")
+        self.X = X.to(device="cuda", dtype=torch.float32).contiguous()
+        self.y = y.to(device="cuda", dtype=torch.float32).contiguous()
+        self.N, self.dim = self.X.shape
+        self.prior_scale = float(prior_scale)
+        if x_dtype not in _lib.X_DTYPES:
+            raise ValueError(f"unknown x_dtype {x_dtype!r}")
+        self.x_dtype = x_dtype
+        self.site_sizes = (("coefs", self.dim, None),)
+
+    def attach(self, handle):
+        lib = _lib.load()
+        _lib.check(lib.nax_model_logistic(handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N,
+                                          self.dim, self.prior_scale,
+                                          _lib.X_DTYPES[self.x_dtype]), "logistic model")
+
+    def postprocess(self, z):
+        return {"coefs": z}
+
+    def algorithmic_flops(self, num_chains):
+        # X.B and X^T.R: 2 * 2 N D flops per chain
+        return 4.0 * self.N * self.dim * num_chains
+
+    def algorithmic_bytes(self, num_chains):
+        elt = {"fp32": 4, "bf16": 2, "fp8": 1}[_canon(self.x_dtype)]
+        return elt * self.N * self.dim + 4.0 * self.N
+
+
+def _canon(x_dtype):
+    return {0: "fp32", 1: "bf16", 2: "fp8"}[_lib.X_DTYPES[x_dtype]]
+
+
+class PotentialFn(Model):
+    """A torch ``potential_fn(z) -> U`` on one latent vector [D]; the engine
+    evaluates it for all chains with ``torch.func.vmap`` + ``grad``.  This is a
+    test and convenience path: the user function runs as torch ops between
+    engine steps."""
+
+    def __init__(self, fn, dim):
+        self.fn = fn
+        self.dim = int(dim)
+        self.site_sizes = ()
+
+    def attach(self, handle):
+        _lib.check(_lib.load().nax_model_external(handle), "external model")
+
+    def postprocess(self, z):
+        return z
+
+    def value_and_grad(self, z):
+        z64 = z.detach().to(torch.float64)
+
+        def single(v):
+            return self.fn(v)
+
+        g, u = torch.func.vmap(torch.func.grad_and_value(single))(z64)
+        return u, g
+
+
+def normal_log_norm(scale):
+    return math.log(math.sqrt(2 * math.pi) * scale)
