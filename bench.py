@@ -1,0 +1,211 @@
+"""Benchmark: 1024-chain NUTS on Bayesian logistic regression (N=2M, D=256).
+
+A "step" is one NUTS iteration of every chain (each chain's full trajectory:
+momentum draw, iterative tree, multinomial proposal) on the synthetic
+2,000,000 x 256 design resident in HBM.  --warmup W untimed iterations run
+first (dual averaging + diagonal Welford on the window schedule of W), then K
+timed sampling iterations.  value = gradient evaluations per second over the
+timed region (useful leapfrog steps summed over chains); min-ESS/s is
+computed from the K timed draws of every chain.  The BF16-MFMA form is the
+measured line; the plain FP32 form (same adapted state, same kernels
+otherwise) is timed beside it, and the CPU baseline (oracle/libcpu_nuts.so,
+a C/OpenMP restatement of the same NUTS path) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {"bf16": 2500.0, "fp32": 157.3, "fp8": 5000.0}  # dense TFLOP/s, MI355X
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True):
+    import torch
+    from numpyro_amd.infer.hmc import Engine
+    eng = Engine(model, C, "NUTS", num_warmup)
+    eng.init(z0, keys)
+    t_start = time.perf_counter()
+    eng.set_progress(lambda ticks, pending: log(
+        f"  {model.x_dtype}: {ticks} gradient ticks, {pending} chains running, "
+        f"{time.perf_counter() - t_start:.0f}s"), every_ticks=progress_ticks(model, C))
+    warm_s = 0.0
+    if state is not None:
+        eng.set_state(state)
+    elif num_warmup > 0:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run(num_warmup)
+        torch.cuda.synchronize()
+        warm_s = time.perf_counter() - t0
+    i0 = int(eng.get_state().i.max())
+    eng.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = eng.run(i0 + steps, collect_fields=("z", "num_steps", "accept_prob", "diverging")
+                  if collect else ("num_steps",), start_idx=i0, thinning=1, size=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, kn = eng.timing(False)
+    return eng, out, wall, warm_s, kms, kn
+
+
+def progress_ticks(model, C):
+    # about one report every 20 s at a few ms per tick for the largest designs
+    work = model.algorithmic_flops(C)
+    return int(max(256, min(65536, 2e16 / max(work, 1.0))))
+
+
+def min_ess(z):
+    from numpyro_amd.diagnostics import effective_sample_size, split_gelman_rubin
+    zz = z.cpu().numpy().astype(np.float64)
+    ess = effective_sample_size(zz)
+    rhat = split_gelman_rubin(zz) if zz.shape[1] >= 4 else np.full(zz.shape[2], np.nan)
+    return float(np.min(ess)), float(np.nanmax(rhat))
+
+
+def cpu_baseline(X, y, z0, keys, budget_s, threads):
+    """Times the C restatement on a bounded sample of the same workload."""
+    from oracle import cpu_baseline as cb
+    Xh = X.cpu().numpy()
+    yh = y.cpu().numpy()
+    C = min(threads, z0.shape[0])
+    z0h = z0[:C].cpu().numpy()
+    kh = np.asarray(keys[:C], np.uint32)
+    # probe: one iteration at max_depth 1 (one leapfrog per chain)
+    t0 = time.perf_counter()
+    _, _, ge = cb.nuts_logistic(Xh, yh, z0h, kh, 0, 1, max_depth=1, step_size=1e-4, adapt=False,
+                                threads=threads)
+    per_eval = (time.perf_counter() - t0) / max(ge, 1) * C
+    depth = int(max(1, min(10, np.floor(np.log2(max(2.0, budget_s / per_eval))))))
+    t0 = time.perf_counter()
+    _, ns, ge = cb.nuts_logistic(Xh, yh, z0h, kh, 0, 1, max_depth=depth, step_size=1e-4,
+                                 adapt=False, threads=threads)
+    wall = time.perf_counter() - t0
+    return ge / wall, C, depth, ge, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1"])
+    ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
+    ap.add_argument("--chains", type=int, default=None)
+    ap.add_argument("--fp32-steps", type=int, default=20,
+                    help="timed iterations of the plain FP32 form (0: skip)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU-baseline work (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); chains "
+              "shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from numpyro_amd import random as nrandom
+    from numpyro_amd import workloads
+    from numpyro_amd.models import LogisticRegression
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
+    N, D, C = cfg["N"], cfg["D"], cfg["chains"]
+    log(f"workload {args.config}: N={N} D={D} chains={C} x_dtype={args.x_dtype}")
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
+    z0 = torch.zeros((C, D), dtype=torch.float32, device="cuda")  # init_to_value(0)
+
+    model = LogisticRegression(X, y, x_dtype=args.x_dtype)
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    ge = eng.grad_evals
+    ns = out["num_steps"].cpu().numpy()
+    div = int(out["diverging"].sum())
+    acc = float(out["accept_prob"].float().mean())
+    mess, rhat = min_ess(out["z"])
+    gps = ge / wall
+    log(f"{args.x_dtype}: warmup {warm_s:.1f}s, timed {wall:.2f}s, grad evals {ge}, "
+        f"{gps:.4g}/s, min ESS {mess:.1f}, max R-hat {rhat:.3f}, mean tree {ns.mean():.1f}, "
+        f"accept {acc:.3f}, divergences {div}")
+    k_avg_ms = kms / max(kn, 1)
+    flops = model.algorithmic_flops(C)
+    achieved = flops / (k_avg_ms * 1e-3) / 1e12
+    peak = PEAKS[args.x_dtype]
+    post_state = eng.get_state()
+
+    fp32 = None
+    if args.fp32_steps > 0 and args.x_dtype != "fp32":
+        m32 = LogisticRegression(X, y, x_dtype="fp32", kernel="mfma")
+        e32, o32, w32, _, k32, n32 = run_form(m32, C, args.warmup, args.fp32_steps, z0, keys,
+                                              state=post_state)
+        me32, rh32 = min_ess(o32["z"]) if args.fp32_steps >= 4 else (float("nan"), float("nan"))
+        fp32 = {"grad_evals_per_s": e32.grad_evals / w32, "min_ess_per_s": me32 / w32,
+                "steps": args.fp32_steps, "kernel_avg_ms": k32 / max(n32, 1),
+                "mean_tree": float(o32["num_steps"].float().mean())}
+        log(f"fp32 form: {fp32}")
+        del e32, m32
+
+    cpu = None
+    if args.cpu_budget > 0:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            cps, cc, depth, cge, cw = cpu_baseline(X, y, z0, keys, args.cpu_budget, threads)
+            cpu = {"value": cps, "unit": "grad evals/s", "cores": threads, "kind": "port",
+                   "sample": f"{cc} chains x 1 NUTS iteration (max_tree_depth {depth}, fixed "
+                             f"step 1e-4, identity mass) on the same {N}x{D} fp32 design: "
+                             f"{cge} grad evals in {cw:.1f}s"}
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:  # the CPU leg is reported, not required
+            log(f"cpu baseline skipped: {e!r}")
+
+    line = {
+        "metric": "min-ESS/sec + grad evals/sec, 1024-chain logistic NUTS, MFMA vs plain FP32",
+        "value": gps,
+        "unit": "grad evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.x_dtype,
+        "data": "synthetic",
+        "config": {"workload": f"logistic NUTS {args.config}: N={N}, D={D}, chains={C}, "
+                               f"X {args.x_dtype}, max_tree_depth 10, target_accept 0.8, "
+                               f"diag mass, init 0", "global_batch": C, "seq_len": N,
+                   "parallelism": "chains"},
+        "min_ess_per_s": mess / wall,
+        "min_ess": mess,
+        "max_rhat": rhat,
+        "mean_tree_size": float(ns.mean()),
+        "mean_accept_prob": acc,
+        "divergences": div,
+        "warmup_s": warm_s,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": model_kernel_name(args.x_dtype), "kernel_avg_ms": k_avg_ms,
+                     "launches": kn, "algorithmic_flops_per_launch": flops},
+        "fp32_form": fp32,
+        "bf16_vs_fp32_min_ess_per_s": (mess / wall) / fp32["min_ess_per_s"]
+        if fp32 and fp32["min_ess_per_s"] > 0 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def model_kernel_name(x_dtype):
+    return {"bf16": "logit_mfma<FormBF16,256>", "fp32": "logit_mfma<FormF32,256>"}[x_dtype]
+
+
+if __name__ == "__main__":
+    main()

@@ -109,6 +109,12 @@ void nax_destroy(nax_sampler* h);
 int nax_model_logistic(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                        int32_t d, double prior_scale, int32_t x_dtype);
 
+/* nax_model_logistic with the kernel choice for fp32 storage: 0 = auto
+ * (matrix cores from 32 chains up), 1 = VALU tile kernel, 2 = fp32 MFMA.
+ * bf16 / fp8 storage always use the matrix cores. */
+int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
+                          int32_t d, double prior_scale, int32_t x_dtype, int32_t kernel);
+
 /* Potential supplied by the caller (a potential_fn, hmc.py:127-130): the
  * engine asks for U and dU at d_z_request, the caller writes them into the
  * gradient buffers, see nax_advance. */
@@ -173,6 +179,12 @@ int nax_random_normal(const uint32_t* d_key /*2*/, int32_t n, int32_t partitiona
  * (hmc.py:744-750).  d_out [C, 2]. */
 int nax_chain_keys(const uint32_t* d_key /*2*/, int32_t num_chains, int32_t partitionable,
                    uint32_t* d_out);
+
+/* Progress report from nax_run's host loop: cb(user, ticks, pending) after
+ * every `every_ticks` gradient ticks (rounded up to the loop's batch of 16);
+ * pending = chains still running.  cb = NULL switches it off. */
+int nax_set_progress(nax_sampler* h, void (*cb)(void* user, int64_t ticks, int32_t pending),
+                     void* user, int64_t every_ticks);
 
 /* Timing of the dominant (gradient) kernel inside nax_run, measured with HIP
  * events on the sampler's stream: total milliseconds and launches since the

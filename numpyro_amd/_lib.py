@@ -46,6 +46,8 @@ class nax_state_view(C.Structure):
     ]
 
 
+PROGRESS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int32)
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "nax_last_error": (C.c_char_p, []),
@@ -53,6 +55,8 @@ SIGNATURES = {
     "nax_destroy": (None, [C.c_void_p]),
     "nax_model_logistic": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
                                      C.c_double, C.c_int32]),
+    "nax_model_logistic_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.c_int32, C.c_double, C.c_int32, C.c_int32]),
     "nax_model_external": (C.c_int, [C.c_void_p]),
     "nax_init": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nax_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(nax_collect),
@@ -73,6 +77,7 @@ SIGNATURES = {
     "nax_random_split": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nax_random_normal": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nax_chain_keys": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "nax_set_progress": (C.c_int, [C.c_void_p, PROGRESS_CB, C.c_void_p, C.c_int64]),
     "nax_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]),
 }

@@ -203,6 +203,7 @@ struct LogisticPotential : Potential {
     const int DP = dch * 64 + 1;
     size_t shm = sizeof(float) * ((size_t)BC * DP + (size_t)BN * DP + BN * BC) + 256 * sizeof(double);
     dim3 grid(RB, ctiles);
+    mark_begin(s);
 #define NAX_LAUNCH(DC)                                                                    \
   hipLaunchKernelGGL((logit_partial_f32<BN, BC, DC>), grid, dim3(256), shm, s, x, y, n, d, z, \
                      ldz, C, RB, wsg, wsu)
@@ -218,6 +219,7 @@ struct LogisticPotential : Potential {
       default: return hipErrorInvalidValue;
     }
 #undef NAX_LAUNCH
+    mark_end(s);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     const double inv_s2 = 1.0 / (prior_scale * prior_scale);
@@ -228,14 +230,23 @@ struct LogisticPotential : Potential {
   }
 };
 
+Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
+                              double prior_scale, bool bf16, hipStream_t s, std::string* err);
+
+// fuse_variant: 0 = auto (matrix cores when num_chains >= 32), 1 = VALU tile
+// kernel, 2 = matrix cores.  bf16 always runs on the matrix cores.
 Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
-                         int x_dtype, int fuse_variant, hipStream_t s, std::string* err) {
+                         int x_dtype, int fuse_variant, int num_chains, hipStream_t s,
+                         std::string* err) {
+  if (x_dtype == NAX_X_BF16) return make_logistic_mfma(x, y, n, d, prior_scale, true, s, err);
   if (x_dtype != NAX_X_F32) {
-    *err = "x_dtype not supported yet";
+    *err = "x_dtype not supported";
     return nullptr;
   }
+  const bool mfma = fuse_variant == 2 || (fuse_variant == 0 && num_chains >= 32 && d <= 256);
+  if (mfma) return make_logistic_mfma(x, y, n, d, prior_scale, false, s, err);
   if (d > 320) {
-    *err = "the fp32 logistic kernel supports dim <= 320";
+    *err = "the fp32 logistic tile kernel supports dim <= 320";
     return nullptr;
   }
   auto* p = new LogisticPotential();

@@ -1,0 +1,376 @@
+// Logistic potential + adjoint on the matrix cores (gfx950 MFMA), C chains.
+//
+// Same mathematics as logistic.hip (numpyro/distributions/discrete.py:244-263,
+// distributions/util.py:317-320, examples/covtype.py:66-71):
+//   L = X Z^T,  R = sigmoid(L) - y,  U_c = sum_n softplus(L) - y L,  G^T = R^T X.
+//
+// Work split: a block is 4 waves; a wave owns 32 chains and walks 32-row tiles
+// of X.  Per tile:
+//   GEMM1  L[32 rows x 32 chains] = sum_k X[., k] Z[., k]   (A = X, B = Z^T)
+//   epilogue on the 16 accumulator values each lane holds
+//   GEMM2  G^T[32 chains x D] += R^T[32 x 32 rows] X[32 rows x D]
+// The accumulator of GEMM1 holds, in lane (c, h) and register v, L at row
+// 8(v/4) + 4h + v%4 of chain c.  GEMM2 takes R^T as its A operand, whose lane
+// (c, h) must supply K-slots 8h..8h+7 of chain c: choosing the K order of
+// GEMM2 so that slot 8h + j of MFMA m is row 16m + 8(j/4) + 4h + j%4 makes that
+// operand exactly accumulator registers 8m..8m+7 -- R never leaves registers.
+// X is stored twice in fragment order (prepared once per model):
+//   A1[T][kk][lane][j] = X[32T + lane%32][KS kk + KS/2 (lane/32) + j]
+//   B2[T][m][dt][lane][j] = X[32T + row(m, lane/32, j)][32 dt + lane%32]
+// so every operand fetch is one coalesced load.  Blocks that walk the same
+// row tiles (same row split, different chain tiles) are placed on the same
+// XCD (blocks are dealt to the 8 XCDs round-robin) to share its L2.
+//
+// Precision forms:
+//   BF16: X in bf16 (RNE); Z enters as a bf16 hi+lo pair (two MFMAs), so the
+//         positions keep ~16 mantissa bits; R as bf16 hi+lo; fp32 accumulate.
+//   FP32: X, Z, R in fp32 on the fp32 MFMA (v_mfma_f32_32x32x2_f32).
+// Energies accumulate in fp64 per lane.  Partials per row split are summed in
+// a fixed order by logit_reduce (logistic.hip).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "nax_device.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ float softplus_f(float x) {
+  return fmaxf(x, 0.0f) + log1pf(__expf(-fabsf(x)));
+}
+__device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// ---- fragment-order preparation -------------------------------------------------
+// KS = K per MFMA (16 for bf16, 2 for fp32); Dp = D rounded up to 32.
+template <typename T, int KS>
+__global__ void prep_a1(const float* __restrict__ X, int64_t N, int D, int Dp, int64_t ntile,
+                        T* __restrict__ out) {
+  const int KK = Dp / KS;
+  const int64_t total = ntile * KK * 64 * (KS / 2);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int j = (int)(i % (KS / 2));
+    int64_t r = i / (KS / 2);
+    int lane = (int)(r % 64);
+    r /= 64;
+    int kk = (int)(r % KK);
+    int64_t t = r / KK;
+    int64_t n = 32 * t + lane % 32;
+    int d = KS * kk + (KS / 2) * (lane / 32) + j;
+    float v = (n < N && d < D) ? X[n * D + d] : 0.0f;
+    out[i] = (T)v;
+  }
+}
+
+template <typename T, int KS>
+__global__ void prep_b2(const float* __restrict__ X, int64_t N, int D, int Dp, int64_t ntile,
+                        T* __restrict__ out) {
+  const int DT = Dp / 32;
+  const int M = 32 / KS;  // MFMAs per 32-row tile in GEMM2
+  const int64_t total = ntile * M * DT * 64 * (KS / 2);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int j = (int)(i % (KS / 2));
+    int64_t r = i / (KS / 2);
+    int lane = (int)(r % 64);
+    r /= 64;
+    int dt = (int)(r % DT);
+    r /= DT;
+    int m = (int)(r % M);
+    int64_t t = r / M;
+    int h = lane / 32;
+    // K-slot (KS/2) h + j of MFMA m <-> accumulator register v = (KS/2) m + j
+    int v = (KS / 2) * m + j;
+    int row = 8 * (v / 4) + 4 * h + (v % 4);
+    int64_t n = 32 * t + row;
+    int d = 32 * dt + lane % 32;
+    float val = (n < N && d < D) ? X[n * D + d] : 0.0f;
+    out[i] = (T)val;
+  }
+}
+
+__global__ void pad_y(const float* __restrict__ y, int64_t N, int64_t Np, float* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Np;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = i < N ? y[i] : 0.0f;
+}
+
+// ---- the fused kernel ----------------------------------------------------------------
+struct FormBF16 {
+  static constexpr int KS = 16;
+  typedef bf16x8 frag;
+  static __device__ __forceinline__ f32x16 mma(frag a, frag b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+struct FormF32 {
+  static constexpr int KS = 2;
+  typedef float frag;
+  static __device__ __forceinline__ f32x16 mma(frag a, frag b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename F, int DP>
+__global__ void __launch_bounds__(256, 1)
+logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __restrict__ B2,
+           const float* __restrict__ ypad, int64_t N, int64_t ntile, const float* __restrict__ Z,
+           int ldz, int D, int C, int CT, int RB, float* __restrict__ wsg,
+           double* __restrict__ wsu) {
+  constexpr int KS = F::KS;
+  constexpr int KK = DP / KS;   // GEMM1 K steps
+  constexpr int DT = DP / 32;   // GEMM2 d tiles
+  constexpr int M2 = 32 / KS;   // GEMM2 MFMAs per d tile and row tile
+  constexpr bool BF = (KS == 16);
+  typedef typename F::frag frag;
+
+  const int wave = threadIdx.x >> 6;
+  const int l = threadIdx.x & 63;
+  const int cl = l & 31, h = l >> 5;
+  // XCD-aware block placement: blocks of one row split share an XCD.
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7;
+  const int slot = bid >> 3;
+  const int ct = slot % CT;
+  const int rb = (slot / CT) * 8 + xcd;
+  if (rb >= RB) return;
+  const int c0 = ct * 128 + wave * 32;
+  const int c = c0 + cl;
+  const bool cvalid = c < C;
+
+  // Z fragments (B operand of GEMM1): lane (c, h) holds Z[c][KS kk + KS/2 h + j]
+  frag zh[KK];
+  frag zlo[BF ? KK : 1];
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk) {
+    if constexpr (BF) {
+      bf16x8 hi, lo;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int d = 16 * kk + 8 * h + j;
+        float v = (cvalid && d < D) ? Z[(size_t)c * ldz + d] : 0.0f;
+        __bf16 b = (__bf16)v;
+        hi[j] = b;
+        lo[j] = (__bf16)(v - (float)b);
+      }
+      zh[kk] = hi;
+      zlo[kk] = lo;
+    } else {
+      int d = 2 * kk + h;
+      zh[kk] = (cvalid && d < D) ? Z[(size_t)c * ldz + d] : 0.0f;
+    }
+  }
+
+  f32x16 g[DT];
+#pragma unroll
+  for (int dt = 0; dt < DT; ++dt)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) g[dt][v] = 0.0f;
+  double u = 0.0;
+
+  for (int64_t t = rb; t < ntile; t += RB) {
+    const frag* a = A1 + (size_t)t * KK * 64;
+    f32x16 L;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) L[v] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      frag af = a[kk * 64 + l];
+      L = F::mma(af, zh[kk], L);
+      if constexpr (BF) L = F::mma(af, zlo[kk], L);
+    }
+    const int64_t nb = 32 * t;
+    float rv[16];
+    float us = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 yq = *(const float4*)(ypad + nb + 8 * q + 4 * h);
+      const float yy[4] = {yq.x, yq.y, yq.z, yq.w};
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int v = 4 * q + m;
+        const int64_t n = nb + 8 * q + 4 * h + m;
+        const float lv = L[v];
+        const bool ok = cvalid && n < N;
+        rv[v] = ok ? sigmoid_f(lv) - yy[m] : 0.0f;
+        us += ok ? softplus_f(lv) - yy[m] * lv : 0.0f;
+      }
+    }
+    u += (double)us;
+    const frag* b = B2 + (size_t)t * M2 * DT * 64;
+    if constexpr (BF) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        bf16x8 rh, rl;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          __bf16 hb = (__bf16)rv[8 * m + j];
+          rh[j] = hb;
+          rl[j] = (__bf16)(rv[8 * m + j] - (float)hb);
+        }
+#pragma unroll
+        for (int dt = 0; dt < DT; ++dt) {
+          frag bf = b[(m * DT + dt) * 64 + l];
+          g[dt] = F::mma(rh, bf, g[dt]);
+          g[dt] = F::mma(rl, bf, g[dt]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+#pragma unroll
+        for (int dt = 0; dt < DT; ++dt) {
+          frag bf = b[(m * DT + dt) * 64 + l];
+          g[dt] = F::mma(rv[m], bf, g[dt]);
+        }
+      }
+    }
+  }
+
+  // G^T partial: register v of tile dt is chain 8(v/4) + 4h + v%4 (relative
+  // to c0), column d = 32 dt + cl.
+#pragma unroll
+  for (int dt = 0; dt < DT; ++dt) {
+    const int d = 32 * dt + cl;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int ci = c0 + 8 * (v / 4) + 4 * h + (v % 4);
+      if (ci < C && d < D) wsg[((size_t)rb * C + ci) * D + d] = g[dt][v];
+    }
+  }
+  const double ut = u + __shfl_xor(u, 32, 64);
+  if (h == 0 && cvalid) wsu[(size_t)rb * C + c] = ut;
+}
+
+// Fixed-order sum of the partials (+ Normal prior), defined in logistic.hip.
+__global__ void logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB,
+                             int C, int D, const float* __restrict__ Z, int ldz, double inv_s2,
+                             double log_norm, double* __restrict__ pe, float* __restrict__ g,
+                             int ldg);
+
+struct LogisticMfma : Potential {
+  int64_t n = 0, ntile = 0;
+  int d = 0, dp = 0;
+  double prior_scale = 1.0;
+  bool bf16 = true;
+  void* a1 = nullptr;
+  void* b2 = nullptr;
+  float* ypad = nullptr;
+  float* wsg = nullptr;
+  double* wsu = nullptr;
+  size_t ws_elems = 0;
+  int num_cu = 256;
+
+  ~LogisticMfma() override {
+    hipFree(a1); hipFree(b2); hipFree(ypad); hipFree(wsg); hipFree(wsu);
+  }
+  int dim() const override { return d; }
+  const char* name() const override { return bf16 ? "logit_mfma_bf16" : "logit_mfma_f32"; }
+  double alg_flops(int C) const override { return 4.0 * (double)n * d * C; }
+  double alg_bytes(int C) const override { return (bf16 ? 2.0 : 4.0) * (double)n * d + 4.0 * n; }
+
+  hipError_t setup(const float* x, const float* y, hipStream_t s) {
+    ntile = (n + 31) / 32;
+    dp = (d + 31) / 32 * 32;
+    const size_t elt = bf16 ? 2 : 4;
+    const size_t per = (size_t)ntile * 32 * dp * elt;
+    hipError_t e = hipMalloc(&a1, per);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&b2, per);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&ypad, sizeof(float) * ntile * 32);
+    if (e != hipSuccess) return e;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) num_cu = prop.multiProcessorCount;
+    const int blocks = 4096;
+    if (bf16) {
+      hipLaunchKernelGGL((prep_a1<__bf16, 16>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)a1);
+      hipLaunchKernelGGL((prep_b2<__bf16, 16>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)b2);
+    } else {
+      hipLaunchKernelGGL((prep_a1<float, 2>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (float*)a1);
+      hipLaunchKernelGGL((prep_b2<float, 2>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (float*)b2);
+    }
+    hipLaunchKernelGGL(pad_y, dim3(1024), dim3(256), 0, s, y, n, ntile * 32, ypad);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+
+  hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe, float* g,
+                  int ldg) override {
+    const int CT = (C + 127) / 128;
+    // one 4-wave block per CU; row splits rounded to a multiple of the 8 XCDs
+    int RB = std::max(8, (num_cu / CT) / 8 * 8);
+    RB = (int)std::min<int64_t>(RB, (ntile + 7) / 8 * 8);
+    const size_t need = (size_t)RB * C * d;
+    if (need > ws_elems) {
+      hipFree(wsg);
+      hipFree(wsu);
+      wsg = nullptr;
+      wsu = nullptr;
+      hipError_t e = hipMalloc(&wsg, sizeof(float) * need);
+      if (e != hipSuccess) return e;
+      e = hipMalloc(&wsu, sizeof(double) * (size_t)RB * C);
+      if (e != hipSuccess) return e;
+      ws_elems = need;
+    }
+    const int nblk = CT * RB;
+    mark_begin(s);
+#define NAX_L(FORM, DPV)                                                                      \
+  hipLaunchKernelGGL((logit_mfma<FORM, DPV>), dim3(nblk), dim3(256), 0, s,                    \
+                     (const FORM::frag*)a1, (const FORM::frag*)b2, ypad, n, ntile, z, ldz, d, C, \
+                     CT, RB, wsg, wsu)
+#define NAX_DP(FORM)                    \
+  switch (dp) {                         \
+    case 32: NAX_L(FORM, 32); break;    \
+    case 64: NAX_L(FORM, 64); break;    \
+    case 96: NAX_L(FORM, 96); break;    \
+    case 128: NAX_L(FORM, 128); break;  \
+    case 160: NAX_L(FORM, 160); break;  \
+    case 192: NAX_L(FORM, 192); break;  \
+    case 224: NAX_L(FORM, 224); break;  \
+    case 256: NAX_L(FORM, 256); break;  \
+    default: return hipErrorInvalidValue; \
+  }
+    if (bf16) { NAX_DP(FormBF16) } else { NAX_DP(FormF32) }
+#undef NAX_DP
+#undef NAX_L
+    mark_end(s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const double inv_s2 = 1.0 / (prior_scale * prior_scale);
+    const double log_norm = std::log(std::sqrt(2.0 * M_PI) * prior_scale);
+    hipLaunchKernelGGL(logit_reduce, dim3(C), dim3(256), 0, s, wsg, wsu, RB, C, d, z, ldz,
+                       inv_s2, log_norm, pe, g, ldg);
+    return hipGetLastError();
+  }
+};
+
+Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
+                              double prior_scale, bool bf16, hipStream_t s, std::string* err) {
+  if (d > 256) {
+    *err = "the MFMA logistic kernels support dim <= 256";
+    return nullptr;
+  }
+  auto* p = new LogisticMfma();
+  p->n = n;
+  p->d = d;
+  p->prior_scale = prior_scale;
+  p->bf16 = bf16;
+  hipError_t e = p->setup(x, y, s);
+  if (e != hipSuccess) {
+    *err = std::string("logistic MFMA setup: ") + hipGetErrorString(e);
+    delete p;
+    return nullptr;
+  }
+  return p;
+}
+
+}  // namespace nax

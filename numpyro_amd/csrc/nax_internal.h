@@ -75,10 +75,16 @@ struct Potential {
   virtual double alg_bytes(int C) const { return 0; }
   virtual double alg_flops(int C) const { return 0; }
   virtual const char* name() const { return "potential"; }
+  // When set, the potential records these events on its stream immediately
+  // before and after its main (dominant) kernel.
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  void mark_begin(hipStream_t s) { if (ev_begin) hipEventRecord(ev_begin, s); }
+  void mark_end(hipStream_t s) { if (ev_end) hipEventRecord(ev_end, s); }
 };
 
 Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
-                         int x_dtype, int fuse_variant, hipStream_t s, std::string* err);
+                         int x_dtype, int fuse_variant, int num_chains, hipStream_t s,
+                         std::string* err);
 
 void set_error(const std::string& msg);
 

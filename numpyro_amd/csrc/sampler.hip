@@ -638,6 +638,9 @@ struct nax_sampler {
   int64_t* gevals = nullptr;   // device
   int* ctr_host = nullptr;     // pinned
   int64_t* gevals_host = nullptr;
+  void (*progress_cb)(void*, int64_t, int32_t) = nullptr;
+  void* progress_user = nullptr;
+  int64_t progress_every = 0;
   bool timing = false;
   double timing_ms = 0.0;
   int64_t timing_n = 0;
@@ -761,12 +764,19 @@ extern "C" void nax_destroy(nax_sampler* h) {
 
 extern "C" int nax_model_logistic(nax_sampler* h, const float* x, const float* y, int64_t n,
                                   int32_t d, double prior_scale, int32_t x_dtype) {
+  return nax_model_logistic_ex(h, x, y, n, d, prior_scale, x_dtype, 0);
+}
+
+extern "C" int nax_model_logistic_ex(nax_sampler* h, const float* x, const float* y, int64_t n,
+                                     int32_t d, double prior_scale, int32_t x_dtype,
+                                     int32_t kernel) {
   if (!h || !x || !y) { set_error("nax_model_logistic: null argument"); return NAX_EINVAL; }
   if (d != h->cfg.D) { set_error("logistic model dim does not match the sampler dim"); return NAX_EINVAL; }
   if (n <= 0) { set_error("logistic model needs n > 0 rows"); return NAX_EINVAL; }
   if (!(prior_scale > 0)) { set_error("prior_scale must be positive"); return NAX_EINVAL; }
   std::string err;
-  Potential* p = make_logistic(x, y, n, d, prior_scale, x_dtype, 0, h->stream, &err);
+  if (kernel < 0 || kernel > 2) { set_error("kernel must be 0 (auto), 1 (tile) or 2 (matrix cores)"); return NAX_EINVAL; }
+  Potential* p = make_logistic(x, y, n, d, prior_scale, x_dtype, kernel, h->cfg.C, h->stream, &err);
   if (!p) { set_error(err); return NAX_EINVAL; }
   delete h->pot;
   h->pot = p;
@@ -782,26 +792,32 @@ extern "C" int nax_model_external(nax_sampler* h) {
   return NAX_OK;
 }
 
-static int eval_potential(nax_sampler* h) {
+// Evaluates the potential for every chain's request.  With timing on, the
+// main kernel of launch k of a batch is bracketed by events ev[2k], ev[2k+1];
+// collect_timing() reads them after the batch has synchronised.
+static int eval_potential(nax_sampler* h, int k = 0) {
   const EngineCfg& e = h->cfg;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->timing) {
-    if (h->ev.size() < 2) {
-      hipEvent_t a, b;
-      NAX_HIP(hipEventCreate(&a));
-      NAX_HIP(hipEventCreate(&b));
-      h->ev = {a, b};
+    if ((int)h->ev.size() < 2 * kBatch) {
+      for (auto x : h->ev) hipEventDestroy(x);
+      h->ev.assign(2 * kBatch, nullptr);
+      for (auto& x : h->ev) NAX_HIP(hipEventCreate(&x));
     }
-    e0 = h->ev[0];
-    e1 = h->ev[1];
-    NAX_HIP(hipEventRecord(e0, h->stream));
+    h->pot->ev_begin = h->ev[2 * k];
+    h->pot->ev_end = h->ev[2 * k + 1];
+  } else {
+    h->pot->ev_begin = h->pot->ev_end = nullptr;
   }
   NAX_HIP(h->pot->eval(h->stream, h->zreq, e.Dp, e.C, h->peout, h->gout, e.Dp));
-  if (h->timing) {
-    NAX_HIP(hipEventRecord(e1, h->stream));
-    NAX_HIP(hipEventSynchronize(e1));
+  h->pot->ev_begin = h->pot->ev_end = nullptr;
+  return NAX_OK;
+}
+
+static int collect_timing(nax_sampler* h, int nlaunch) {
+  if (!h->timing) return NAX_OK;
+  for (int k = 0; k < nlaunch; ++k) {
     float ms = 0;
-    NAX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    NAX_HIP(hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]));
     h->timing_ms += ms;
     h->timing_n += 1;
   }
@@ -871,16 +887,22 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
   if (h->external || !h->pot) { set_error("nax_run needs a built-in model; use nax_advance"); return NAX_ESTATE; }
   Collect col = to_collect(c);
   NAX_HIP(hipMemsetAsync(h->gevals, 0, sizeof(int64_t), h->stream));
+  int64_t ticks = 0;
   for (;;) {
     hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, h->stream, h->ctr, kBatch);
     for (int k = 0; k < kBatch; ++k) {
       int rc = launch_advance(h, col, target, k);
       if (rc) return rc;
-      rc = eval_potential(h);
+      rc = eval_potential(h, k);
       if (rc) return rc;
     }
     NAX_HIP(hipMemcpyAsync(h->ctr_host, h->ctr + kBatch - 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     NAX_HIP(hipStreamSynchronize(h->stream));
+    int rc = collect_timing(h, kBatch);
+    if (rc) return rc;
+    ticks += kBatch;
+    if (h->progress_cb && h->progress_every > 0 && ticks % h->progress_every == 0)
+      h->progress_cb(h->progress_user, ticks, h->ctr_host[0]);
     if (h->ctr_host[0] == 0) break;
   }
   if (gevals) {
@@ -1058,5 +1080,14 @@ extern "C" int nax_exchange(nax_sampler* h, float* z_out, const double* pe_in, c
     NAX_HIP(hipMemcpy2DAsync(h->gout, sizeof(float) * e.Dp, g_in, sizeof(float) * e.D,
                              sizeof(float) * e.D, e.C, hipMemcpyDeviceToDevice, h->stream));
   NAX_HIP(hipStreamSynchronize(h->stream));
+  return NAX_OK;
+}
+
+extern "C" int nax_set_progress(nax_sampler* h, void (*cb)(void*, int64_t, int32_t), void* user,
+                                int64_t every_ticks) {
+  if (!h) { set_error("null handle"); return NAX_EINVAL; }
+  h->progress_cb = cb;
+  h->progress_user = user;
+  h->progress_every = every_ticks > 0 ? (every_ticks + kBatch - 1) / kBatch * kBatch : 0;
   return NAX_OK;
 }

@@ -172,6 +172,17 @@ class Engine:
             out["diverging"] = out["diverging"].bool()
         return out
 
+    def set_progress(self, fn, every_ticks=2048):
+        """fn(ticks, pending) is called from the run loop every `every_ticks`
+        gradient evaluations; None switches it off."""
+        if fn is None:
+            self._progress = None
+            _lib.check(self.lib.nax_set_progress(self.h, _lib.PROGRESS_CB(), None, 0), "progress")
+            return
+        self._progress = _lib.PROGRESS_CB(lambda user, ticks, pending: fn(ticks, pending))
+        _lib.check(self.lib.nax_set_progress(self.h, self._progress, None, int(every_ticks)),
+                   "progress")
+
     def timing(self, enable):
         ms, n = C.c_double(), C.c_int64()
         _lib.check(self.lib.nax_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)),

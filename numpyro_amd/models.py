@@ -47,7 +47,7 @@ class LogisticRegression(Model):
     picks the storage the gradient kernels read: ``"fp32"`` (plain), ``"bf16"``
     (MFMA) or ``"fp8"`` (MFMA, e4m3 with per-column scales)."""
 
-    def __init__(self, X, y, prior_scale=1.0, x_dtype="fp32"):
+    def __init__(self, X, y, prior_scale=1.0, x_dtype="fp32", kernel="auto"):
         X = torch.as_tensor(X)
         y = torch.as_tensor(y)
         if X.ndim != 2 or y.ndim != 1 or y.shape[0] != X.shape[0]:
@@ -59,13 +59,18 @@ class LogisticRegression(Model):
         if x_dtype not in _lib.X_DTYPES:
             raise ValueError(f"unknown x_dtype {x_dtype!r}")
         self.x_dtype = x_dtype
+        if kernel not in ("auto", "tile", "mfma"):
+            raise ValueError("kernel must be 'auto', 'tile' or 'mfma'")
+        self.kernel = kernel
         self.site_sizes = (("coefs", self.dim, None),)
 
     def attach(self, handle):
         lib = _lib.load()
-        _lib.check(lib.nax_model_logistic(handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N,
-                                          self.dim, self.prior_scale,
-                                          _lib.X_DTYPES[self.x_dtype]), "logistic model")
+        kern = {"auto": 0, "tile": 1, "mfma": 2}[self.kernel]
+        _lib.check(lib.nax_model_logistic_ex(handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N,
+                                             self.dim, self.prior_scale,
+                                             _lib.X_DTYPES[self.x_dtype], kern),
+                   "logistic model")
 
     def postprocess(self, z):
         return {"coefs": z}

@@ -1,0 +1,49 @@
+"""Synthetic logistic-regression workloads (generated on the GPU).
+
+Shapes follow the benchmark configurations:
+  c1: X in R^{1000 x 25} N(0,1) + intercept, beta* ~ N(0,1); 4 chains.
+  c2: N=581,012, D=54 (10 columns N(0,1), 44 binary Bernoulli(0.1)) +
+      intercept, beta* ~ N(0, 0.5^2); 256 chains.
+  c5: N=2,000,000, D=256, X ~ N(0,1) with 51 (20%) columns scaled x10,
+      beta* ~ N(0, 0.1^2); 1024 chains.
+y ~ Bernoulli(sigmoid(X beta*)).  The draws use torch's generator on the
+device, seeded per configuration, so a 2M x 256 design is built in HBM in
+well under a second.
+"""
+import torch
+
+CONFIGS = {
+    "c1": dict(N=1000, D=26, chains=4, seed=1),
+    "c2": dict(N=581_012, D=55, chains=256, seed=2),
+    "c5": dict(N=2_000_000, D=256, chains=1024, seed=5),
+}
+
+
+def make(name, device="cuda", N=None, chains=None):
+    cfg = dict(CONFIGS[name])
+    if N is not None:
+        cfg["N"] = int(N)
+    if chains is not None:
+        cfg["chains"] = int(chains)
+    g = torch.Generator(device=device)
+    g.manual_seed(cfg["seed"])
+    N, D = cfg["N"], cfg["D"]
+    if name == "c1":
+        X = torch.randn((N, D - 1), generator=g, device=device)
+        X = torch.cat([X, torch.ones((N, 1), device=device)], 1)
+        beta = torch.randn(D, generator=g, device=device)
+    elif name == "c2":
+        Xc = torch.randn((N, 10), generator=g, device=device)
+        Xb = (torch.rand((N, 44), generator=g, device=device) < 0.1).float()
+        X = torch.cat([Xc, Xb, torch.ones((N, 1), device=device)], 1)
+        beta = 0.5 * torch.randn(D, generator=g, device=device)
+    elif name == "c5":
+        X = torch.randn((N, D), generator=g, device=device)
+        cols = torch.randperm(D, generator=g, device=device)[:51]
+        X[:, cols] *= 10.0
+        beta = 0.1 * torch.randn(D, generator=g, device=device)
+    else:
+        raise ValueError(name)
+    logits = X @ beta
+    y = (torch.rand(N, generator=g, device=device) < torch.sigmoid(logits)).float()
+    return X.contiguous(), y.contiguous(), beta, cfg
