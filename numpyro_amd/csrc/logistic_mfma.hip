@@ -128,9 +128,18 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   constexpr int M2 = 32 / KS;   // GEMM2 MFMAs per d tile and row tile
   constexpr bool BF = (KS == 16);
   typedef typename F::frag frag;
+  // one row tile in LDS: A1 part, B2 part, 32 labels
+  constexpr int ABYTES = KK * 64 * (int)sizeof(frag);
+  constexpr int BBYTES = M2 * DT * 64 * (int)sizeof(frag);
+  constexpr int TBYTES = ABYTES + BBYTES + 128;
+  constexpr int PF = (ABYTES + BBYTES) / (256 * 16);   // 16-byte chunks per thread
+  constexpr int PA = ABYTES / 16;                      // chunks in the A1 part
+  static_assert((ABYTES + BBYTES) % (256 * 16) == 0, "tile must split into 16-byte chunks");
+  __shared__ __attribute__((aligned(16))) char lds[2][TBYTES];
 
-  const int wave = threadIdx.x >> 6;
-  const int l = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int l = tid & 63;
   const int cl = l & 31, h = l >> 5;
   // XCD-aware block placement: blocks of one row split share an XCD.
   const int bid = blockIdx.x;
@@ -138,7 +147,7 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   const int slot = bid >> 3;
   const int ct = slot % CT;
   const int rb = (slot / CT) * 8 + xcd;
-  if (rb >= RB) return;
+  if (rb >= RB) return;  // uniform over the block
   const int c0 = ct * 128 + wave * 32;
   const int c = c0 + cl;
   const bool cvalid = c < C;
@@ -173,36 +182,69 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
     for (int v = 0; v < 16; ++v) g[dt][v] = 0.0f;
   double u = 0.0;
 
-  for (int64_t t = rb; t < ntile; t += RB) {
-    const frag* a = A1 + (size_t)t * KK * 64;
+  // ---- tile staging: global -> registers (prefetch) -> LDS ----
+  uint4 pf[PF];
+  float4 pfy = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto load_tile = [&](int64_t t) {
+    const uint4* ga = (const uint4*)(A1 + (size_t)t * KK * 64);
+    const uint4* gb = (const uint4*)(B2 + (size_t)t * M2 * DT * 64);
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int idx = tid + 256 * p;
+      pf[p] = idx < PA ? ga[idx] : gb[idx - PA];
+    }
+    if (tid < 8) pfy = ((const float4*)(ypad + 32 * t))[tid];
+  };
+  auto store_tile = [&](int buf) {
+    uint4* s4 = (uint4*)lds[buf];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) s4[tid + 256 * p] = pf[p];
+    if (tid < 8) ((float4*)(lds[buf] + ABYTES + BBYTES))[tid] = pfy;
+  };
+
+  int64_t t = rb;
+  int cur = 0;
+  if (t < ntile) {
+    load_tile(t);
+    store_tile(0);
+  }
+  __syncthreads();
+  for (; t < ntile; t += RB) {
+    const bool has_next = t + RB < ntile;
+    if (has_next) load_tile(t + RB);
+    const frag* la = (const frag*)lds[cur];
+    const frag* lb = (const frag*)(lds[cur] + ABYTES);
+    const float* ly = (const float*)(lds[cur] + ABYTES + BBYTES);
+
+    // GEMM1: L = X_tile Z^T
     f32x16 L;
 #pragma unroll
     for (int v = 0; v < 16; ++v) L[v] = 0.0f;
 #pragma unroll
     for (int kk = 0; kk < KK; ++kk) {
-      frag af = a[kk * 64 + l];
+      frag af = la[kk * 64 + l];
       L = F::mma(af, zh[kk], L);
       if constexpr (BF) L = F::mma(af, zlo[kk], L);
     }
+    // epilogue: register v holds row 8(v/4) + 4h + v%4
     const int64_t nb = 32 * t;
     float rv[16];
     float us = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const float4 yq = *(const float4*)(ypad + nb + 8 * q + 4 * h);
+      const float4 yq = *(const float4*)(ly + 8 * q + 4 * h);
       const float yy[4] = {yq.x, yq.y, yq.z, yq.w};
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int v = 4 * q + m;
-        const int64_t n = nb + 8 * q + 4 * h + m;
+        const bool ok = cvalid && (nb + 8 * q + 4 * h + m) < N;
         const float lv = L[v];
-        const bool ok = cvalid && n < N;
         rv[v] = ok ? sigmoid_f(lv) - yy[m] : 0.0f;
         us += ok ? softplus_f(lv) - yy[m] * lv : 0.0f;
       }
     }
     u += (double)us;
-    const frag* b = B2 + (size_t)t * M2 * DT * 64;
+    // GEMM2: G^T += R^T X_tile
     if constexpr (BF) {
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
@@ -215,7 +257,7 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
         }
 #pragma unroll
         for (int dt = 0; dt < DT; ++dt) {
-          frag bf = b[(m * DT + dt) * 64 + l];
+          frag bf = lb[(m * DT + dt) * 64 + l];
           g[dt] = F::mma(rh, bf, g[dt]);
           g[dt] = F::mma(rl, bf, g[dt]);
         }
@@ -225,11 +267,14 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
       for (int m = 0; m < 16; ++m) {
 #pragma unroll
         for (int dt = 0; dt < DT; ++dt) {
-          frag bf = b[(m * DT + dt) * 64 + l];
+          frag bf = lb[(m * DT + dt) * 64 + l];
           g[dt] = F::mma(rv[m], bf, g[dt]);
         }
       }
     }
+    if (has_next) store_tile(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 
   // G^T partial: register v of tile dt is chain 8(v/4) + 4h + v%4 (relative

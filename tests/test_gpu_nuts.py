@@ -75,7 +75,7 @@ def test_nuts_chain_matches_oracle(seed, adapt):
     assert np.array_equal(gns, ons), (gns, ons)
     horizon = 25 if adapt else num_warmup + num_samples
     np.testing.assert_allclose(gz[:, :horizon], oz[:, :horizon], rtol=1e-3, atol=1e-3)
-    np.testing.assert_allclose(gpe[:, :horizon], ope[:, :horizon], rtol=1e-5)
+    np.testing.assert_allclose(gpe[:, :horizon], ope[:, :horizon], rtol=1e-4 if adapt else 1e-5)
     np.testing.assert_allclose(gap[:, :horizon], oap[:, :horizon], rtol=1e-3, atol=1e-4)
 
 
