@@ -130,7 +130,7 @@ logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int6
 __global__ void __launch_bounds__(256)
 logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB, int C,
              int D, const float* __restrict__ Z, int ldz, double inv_s2, double log_norm,
-             double* __restrict__ pe, float* __restrict__ g, int ldg) {
+             double u_offset, double* __restrict__ pe, float* __restrict__ g, int ldg) {
   const int c = blockIdx.x;
   const int t = threadIdx.x;
   __shared__ double red[256];
@@ -151,7 +151,7 @@ logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int 
   if (t == 0) {
     double u = 0.0;
     for (int r = 0; r < RB; ++r) u += wsu[(size_t)r * C + c];
-    pe[c] = u + red[0];
+    pe[c] = u + red[0] + u_offset;
   }
 }
 
@@ -225,7 +225,7 @@ struct LogisticPotential : Potential {
     const double inv_s2 = 1.0 / (prior_scale * prior_scale);
     const double log_norm = std::log(std::sqrt(2.0 * M_PI) * prior_scale);
     hipLaunchKernelGGL(logit_reduce, dim3(C), dim3(256), 0, s, wsg, wsu, RB, C, d, z, ldz,
-                       inv_s2, log_norm, pe, g, ldg);
+                       inv_s2, log_norm, 0.0, pe, g, ldg);
     return hipGetLastError();
   }
 };

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <string>
 
 #include "nax_device.h"
@@ -44,6 +45,19 @@ __device__ __forceinline__ float softplus_f(float x) {
   return fmaxf(x, 0.0f) + log1pf(__expf(-fabsf(x)));
 }
 __device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// log-likelihood terms of one (row, chain): r = sigmoid(l) - y and
+// u = softplus(l) - y l, from e = exp(-|l|) with the hardware exp2 / rcp /
+// log2 (each within ~1 ulp; absolute error of u ~1e-7 per row).
+__device__ __forceinline__ void logit_terms(float l, float y, float& r, float& u) {
+  const float e = __builtin_amdgcn_exp2f(-fabsf(l) * 1.44269504088896341f);
+  const float d = 1.0f + e;
+  const float inv = __builtin_amdgcn_rcpf(d);
+  const float s = l >= 0.0f ? inv : e * inv;
+  r = s - y;
+  u = fmaf(__builtin_amdgcn_logf(d), 0.693147180559945309f, fmaxf(l, 0.0f) - y * l);
+}
+
 
 // ---- fragment-order preparation -------------------------------------------------
 // KS = K per MFMA (16 for bf16, 2 for fp32); Dp = D rounded up to 32.
@@ -67,7 +81,7 @@ __global__ void prep_a1(const float* __restrict__ X, int64_t N, int D, int Dp, i
   }
 }
 
-template <typename T, int KS>
+template <typename T, int KS, bool VIA_BF16 = false>
 __global__ void prep_b2(const float* __restrict__ X, int64_t N, int D, int Dp, int64_t ntile,
                         T* __restrict__ out) {
   const int DT = Dp / 32;
@@ -90,6 +104,7 @@ __global__ void prep_b2(const float* __restrict__ X, int64_t N, int D, int Dp, i
     int64_t n = 32 * t + row;
     int d = 32 * dt + lane % 32;
     float val = (n < N && d < D) ? X[n * D + d] : 0.0f;
+    if (VIA_BF16) val = (float)(__bf16)val;  // the same quantised X as the A1 copy
     out[i] = (T)val;
   }
 }
@@ -97,7 +112,7 @@ __global__ void prep_b2(const float* __restrict__ X, int64_t N, int D, int Dp, i
 __global__ void pad_y(const float* __restrict__ y, int64_t N, int64_t Np, float* __restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = i < N ? y[i] : 0.0f;
+    out[i] = i < N ? y[i] : 0.5f;
 }
 
 // ---- the fused kernel ----------------------------------------------------------------
@@ -226,8 +241,8 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
       L = F::mma(af, zh[kk], L);
       if constexpr (BF) L = F::mma(af, zlo[kk], L);
     }
-    // epilogue: register v holds row 8(v/4) + 4h + v%4
-    const int64_t nb = 32 * t;
+    // epilogue: register v holds row 8(v/4) + 4h + v%4 (padded rows give
+    // r = 0 and u = fp32(ln 2), removed by the reduction)
     float rv[16];
     float us = 0.0f;
 #pragma unroll
@@ -236,11 +251,9 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
       const float yy[4] = {yq.x, yq.y, yq.z, yq.w};
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        const int v = 4 * q + m;
-        const bool ok = cvalid && (nb + 8 * q + 4 * h + m) < N;
-        const float lv = L[v];
-        rv[v] = ok ? sigmoid_f(lv) - yy[m] : 0.0f;
-        us += ok ? softplus_f(lv) - yy[m] * lv : 0.0f;
+        float uu;
+        logit_terms(L[4 * q + m], yy[m], rv[4 * q + m], uu);
+        us += uu;
       }
     }
     u += (double)us;
@@ -292,11 +305,243 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   if (h == 0 && cvalid) wsu[(size_t)rb * C + c] = ut;
 }
 
+// ---- pipelined BF16 kernel --------------------------------------------------------
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+
+// Software-pipelined form of logit_mfma for bf16 X.  The block's row tiles
+// i = 0..n-1 (tile rb + i RB) rotate through a 4-slot LDS ring filled by
+// direct global->LDS loads (no staging registers) issued two tiles ahead.
+// Iteration i runs, as one branch-free basic block, the epilogue of tile i
+// (VALU) interleaved in source order with GEMM1 of tile i+1 and GEMM2 of
+// tile i-1 (MFMA), so the matrix cores and the VALU work on different tiles
+// at once.  GEMM2 runs in fp16: X (bf16 values) is exact in fp16 and R gets
+// 11 mantissa bits in a single MFMA.
+template <int DP>
+__global__ void __launch_bounds__(256, 1)
+logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
+                const float* __restrict__ ypad, int64_t N, int64_t ntile,
+                const float* __restrict__ Z, int ldz, int D, int C, int CT, int RB,
+                float* __restrict__ wsg, double* __restrict__ wsu) {
+  constexpr int KK = DP / 16;
+  constexpr int DT = DP / 32;
+  constexpr int ABYTES = KK * 64 * 16;
+  constexpr int BBYTES = 2 * DT * 64 * 16;
+  constexpr int TBYTES = ABYTES + BBYTES + 128;
+  constexpr int PF = (ABYTES + BBYTES) / (256 * 16);
+  constexpr int PA = ABYTES / 16;
+  static_assert((ABYTES + BBYTES) % (256 * 16) == 0, "tile must split into 16-byte chunks");
+  static_assert(PA % 64 == 0, "a wave's 64 chunks must not straddle A1 and B2");
+  __shared__ __attribute__((aligned(16))) char lds[4][TBYTES];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int l = tid & 63;
+  const int cl = l & 31, h = l >> 5;
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7;
+  const int slot = bid >> 3;
+  const int ct = slot % CT;
+  const int rb = (slot / CT) * 8 + xcd;
+  if (rb >= RB) return;  // uniform over the block
+  const int c0 = ct * 128 + wave * 32;
+  const int c = c0 + cl;
+  const bool cvalid = c < C;
+
+  bf16x8 zh[KK], zlo[KK];
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk) {
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int d = 16 * kk + 8 * h + j;
+      const float v = (cvalid && d < D) ? Z[(size_t)c * ldz + d] : 0.0f;
+      const __bf16 b = (__bf16)v;
+      hi[j] = b;
+      lo[j] = (__bf16)(v - (float)b);
+    }
+    zh[kk] = hi;
+    zlo[kk] = lo;
+  }
+  f32x16 g[DT];
+#pragma unroll
+  for (int dt = 0; dt < DT; ++dt)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) g[dt][v] = 0.0f;
+  double u = 0.0;
+
+  const int64_t n = rb < ntile ? (ntile - 1 - rb) / RB + 1 : 0;
+  // Tile i (clamped to the block's last tile, so always in bounds) goes to
+  // slot i & 3.  Lane l of wave w loads chunk tid + 256 p to LDS offset
+  // 16 (tid + 256 p): each wave's 64 chunks are contiguous, as the
+  // LDS-direct load (LDS address = wave base + 16 lane) requires.
+  auto load_tile = [&](int64_t i) {
+    const int64_t t = rb + (i < n ? i : n - 1) * (int64_t)RB;
+    const uint4* ga = (const uint4*)(A1 + (size_t)t * KK * 64);
+    const uint4* gb = (const uint4*)(B2 + (size_t)t * 2 * DT * 64);
+    char* base = lds[i & 3];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int idx = tid + 256 * p;
+      const uint4* src = idx < PA ? ga + idx : gb + (idx - PA);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)src,
+          (__attribute__((address_space(3))) void*)(base + 16 * (256 * p + 64 * wave)), 16, 0, 0);
+    }
+    if (tid < 8)
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(ypad + 32 * t + 4 * tid),
+          (__attribute__((address_space(3))) void*)(base + ABYTES + BBYTES), 16, 0, 0);
+  };
+  auto wait_tiles = [&]() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  };
+  auto gemm1 = [&](int64_t i) {
+    const bf16x8* la = (const bf16x8*)lds[i & 3];
+    f32x16 L;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) L[v] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      const bf16x8 af = la[kk * 64 + l];
+      L = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zh[kk], L, 0, 0, 0);
+      L = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zlo[kk], L, 0, 0, 0);
+    }
+    return L;
+  };
+  auto gemm2 = [&](int64_t i, f16x8 r0, f16x8 r1) {
+    const f16x8* lb = (const f16x8*)(lds[i & 3] + ABYTES);
+#pragma unroll
+    for (int dt = 0; dt < DT; ++dt) {
+      g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(r0, lb[dt * 64 + l], g[dt], 0, 0, 0);
+      g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(r1, lb[(DT + dt) * 64 + l], g[dt], 0, 0, 0);
+    }
+  };
+  // Rows past N have X = 0 and y = 1/2, so r = 0 exactly and u = fp32(ln 2)
+  // (removed by the reduction); chains past C have Z = 0 and are never
+  // written.  No per-element masking is needed.  Register v of L holds row
+  // 8(v/4) + 4h + v%4.
+  auto load_y = [&](int64_t i, float* yv) {
+    const float* ly = (const float*)(lds[i & 3] + ABYTES + BBYTES);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 yq = *(const float4*)(ly + 8 * q + 4 * h);
+      yv[4 * q] = yq.x;
+      yv[4 * q + 1] = yq.y;
+      yv[4 * q + 2] = yq.z;
+      yv[4 * q + 3] = yq.w;
+    }
+  };
+  auto epilogue = [&](int64_t i, const f32x16& L, f16x8& r0, f16x8& r1) {
+    float yv[16];
+    load_y(i, yv);
+    float us = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      float r, uu;
+      logit_terms(L[v], yv[v], r, uu);
+      us += uu;
+      if (v < 8) r0[v] = (_Float16)r;
+      else r1[v - 8] = (_Float16)r;
+    }
+    u += (double)us;
+  };
+  // steady-state step: epilogue(i) || GEMM1(i+1) || GEMM2(i-1), interleaved
+  // element by element in source order.
+  auto fused_step = [&](int64_t i, const f32x16& Lc, f16x8 rp0, f16x8 rp1, f16x8& r0,
+                        f16x8& r1) {
+    const bf16x8* la = (const bf16x8*)lds[(i + 1) & 3];
+    const f16x8* lb = (const f16x8*)(lds[(i - 1) & 3] + ABYTES);
+    float yv[16];
+    load_y(i, yv);
+    f32x16 Ln;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) Ln[v] = 0.0f;
+    float us = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+#pragma unroll
+      for (int kk = e * KK / 16; kk < (e + 1) * KK / 16; ++kk) {
+        const bf16x8 af = la[kk * 64 + l];
+        Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zh[kk], Ln, 0, 0, 0);
+        Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zlo[kk], Ln, 0, 0, 0);
+      }
+#pragma unroll
+      for (int j = e * 2 * DT / 16; j < (e + 1) * 2 * DT / 16; ++j) {
+        const int m2 = j / DT, dt = j % DT;
+        g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(m2 ? rp1 : rp0, lb[(m2 * DT + dt) * 64 + l],
+                                                       g[dt], 0, 0, 0);
+      }
+      float r, uu;
+      logit_terms(Lc[e], yv[e], r, uu);
+      us += uu;
+      if (e < 8) r0[e] = (_Float16)r;
+      else r1[e - 8] = (_Float16)r;
+    }
+    u += (double)us;
+    return Ln;
+  };
+
+  if (n > 0) {
+    load_tile(0);
+    load_tile(1);
+    wait_tiles();
+    load_tile(2);
+    f32x16 Lc = gemm1(0);
+    f16x8 rp0, rp1;
+    {  // i = 0: epilogue(0) + gemm1(1)
+      f16x8 r0, r1;
+      epilogue(0, Lc, r0, r1);
+      if (n > 1) Lc = gemm1(1);
+      rp0 = r0;
+      rp1 = r1;
+      wait_tiles();
+    }
+    for (int64_t i = 1; i + 1 < n; ++i) {  // steady state, branch-free body
+      load_tile(i + 2);
+      f16x8 r0, r1;
+      Lc = fused_step(i, Lc, rp0, rp1, r0, r1);
+      rp0 = r0;
+      rp1 = r1;
+      wait_tiles();
+    }
+    if (n > 1) {  // i = n-1: epilogue(n-1) + gemm2(n-2)
+      f16x8 r0, r1;
+      epilogue(n - 1, Lc, r0, r1);
+      gemm2(n - 2, rp0, rp1);
+      rp0 = r0;
+      rp1 = r1;
+    }
+    gemm2(n - 1, rp0, rp1);
+  }
+
+#pragma unroll
+  for (int dt = 0; dt < DT; ++dt) {
+    const int d = 32 * dt + cl;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int ci = c0 + 8 * (v / 4) + 4 * h + (v % 4);
+      if (ci < C && d < D) wsg[((size_t)rb * C + ci) * D + d] = g[dt][v];
+    }
+  }
+  const double ut = u + __shfl_xor(u, 32, 64);
+  if (h == 0 && cvalid) wsu[(size_t)rb * C + c] = ut;
+}
+
+__global__ void abs_max_kernel(const float* __restrict__ X, int64_t n, unsigned* __restrict__ out) {
+  float m = 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(X[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
 // Fixed-order sum of the partials (+ Normal prior), defined in logistic.hip.
 __global__ void logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB,
                              int C, int D, const float* __restrict__ Z, int ldz, double inv_s2,
-                             double log_norm, double* __restrict__ pe, float* __restrict__ g,
-                             int ldg);
+                             double log_norm, double u_offset, double* __restrict__ pe,
+                             float* __restrict__ g, int ldg);
 
 struct LogisticMfma : Potential {
   int64_t n = 0, ntile = 0;
@@ -336,8 +581,21 @@ struct LogisticMfma : Potential {
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) num_cu = prop.multiProcessorCount;
     const int blocks = 4096;
     if (bf16) {
+      unsigned* dmax = nullptr;
+      e = hipMallocAsync((void**)&dmax, sizeof(unsigned), s);
+      if (e != hipSuccess) return e;
+      hipMemsetAsync(dmax, 0, sizeof(unsigned), s);
+      hipLaunchKernelGGL(abs_max_kernel, dim3(2048), dim3(256), 0, s, x, n * (int64_t)d, dmax);
+      unsigned hmax = 0;
+      hipMemcpyAsync(&hmax, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+      hipFreeAsync(dmax, s);
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+      float fmax_x;
+      memcpy(&fmax_x, &hmax, 4);
+      if (!(fmax_x < 60000.0f)) return hipErrorInvalidValue;  // fp16 GEMM2 range
       hipLaunchKernelGGL((prep_a1<__bf16, 16>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)a1);
-      hipLaunchKernelGGL((prep_b2<__bf16, 16>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)b2);
+      hipLaunchKernelGGL((prep_b2<_Float16, 16, true>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (_Float16*)b2);
     } else {
       hipLaunchKernelGGL((prep_a1<float, 2>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (float*)a1);
       hipLaunchKernelGGL((prep_b2<float, 2>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (float*)b2);
@@ -384,7 +642,23 @@ struct LogisticMfma : Potential {
     case 256: NAX_L(FORM, 256); break;  \
     default: return hipErrorInvalidValue; \
   }
-    if (bf16) { NAX_DP(FormBF16) } else { NAX_DP(FormF32) }
+#define NAX_P(DPV)                                                                             \
+  hipLaunchKernelGGL((logit_bf16_pipe<DPV>), dim3(nblk), dim3(256), 0, s, (const bf16x8*)a1,   \
+                     (const f16x8*)b2, ypad, n, ntile, z, ldz, d, C, CT, RB, wsg, wsu)
+    if (bf16) {
+      switch (dp) {
+        case 32: NAX_P(32); break;
+        case 64: NAX_P(64); break;
+        case 96: NAX_P(96); break;
+        case 128: NAX_P(128); break;
+        case 160: NAX_P(160); break;
+        case 192: NAX_P(192); break;
+        case 224: NAX_P(224); break;
+        case 256: NAX_P(256); break;
+        default: return hipErrorInvalidValue;
+      }
+    } else { NAX_DP(FormF32) }
+#undef NAX_P
 #undef NAX_DP
 #undef NAX_L
     mark_end(s);
@@ -392,8 +666,10 @@ struct LogisticMfma : Potential {
     if (e != hipSuccess) return e;
     const double inv_s2 = 1.0 / (prior_scale * prior_scale);
     const double log_norm = std::log(std::sqrt(2.0 * M_PI) * prior_scale);
+    // padded rows each added fp32(ln 2) to every chain's energy
+    const double pad_u = (double)(ntile * 32 - n) * (double)0.693147180559945309f;
     hipLaunchKernelGGL(logit_reduce, dim3(C), dim3(256), 0, s, wsg, wsu, RB, C, d, z, ldz,
-                       inv_s2, log_norm, pe, g, ldg);
+                       inv_s2, log_norm, -pad_u, pe, g, ldg);
     return hipGetLastError();
   }
 };
@@ -411,7 +687,8 @@ Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
   p->bf16 = bf16;
   hipError_t e = p->setup(x, y, s);
   if (e != hipSuccess) {
-    *err = std::string("logistic MFMA setup: ") + hipGetErrorString(e);
+    *err = std::string("logistic MFMA setup: ") + hipGetErrorString(e) +
+           (bf16 && e == hipErrorInvalidValue ? " (the bf16 form needs max|X| < 60000)" : "");
     delete p;
     return nullptr;
   }
