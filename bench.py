@@ -97,15 +97,15 @@ def cpu_baseline(X, y, z0, keys, budget_s, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=150)
     ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1"])
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
     ap.add_argument("--chains", type=int, default=None)
-    ap.add_argument("--fp32-steps", type=int, default=20,
+    ap.add_argument("--fp32-steps", type=int, default=8,
                     help="timed iterations of the plain FP32 form (0: skip)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0,
+    ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of CPU-baseline work (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
