@@ -192,7 +192,7 @@ def main():
         "divergences": div,
         "warmup_s": warm_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic_bytes(args, N, D, C),
                      "kernel": model_kernel_name(args.x_dtype), "kernel_avg_ms": k_avg_ms,
                      "launches": kn, "algorithmic_flops_per_launch": flops},
         "fp32_form": fp32,
@@ -204,7 +204,17 @@ def main():
 
 
 def model_kernel_name(x_dtype):
-    return {"bf16": "logit_mfma<FormBF16,256>", "fp32": "logit_mfma<FormF32,256>"}[x_dtype]
+    return {"bf16": "logit_bf16_pipe<256>", "fp32": "logit_mfma<FormF32,256>"}[x_dtype]
+
+
+# HBM bytes per launch of the main kernel measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy: x2.0 / x1.0), see
+# profiles/logit_bf16_pipe_c5.md.  Only known for the measured workload.
+_MEASURED_TRAFFIC = {("bf16", 2_000_000, 256, 1024): 2_016_666 * 1024 + 33_024 * 1024}
+
+
+def traffic_bytes(args, N, D, C):
+    return _MEASURED_TRAFFIC.get((args.x_dtype, N, D, C))
 
 
 if __name__ == "__main__":
