@@ -99,7 +99,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=150)
-    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1"])
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3"])
+    ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
     ap.add_argument("--chains", type=int, default=None)
@@ -119,6 +120,10 @@ def main():
     from numpyro_amd.models import LogisticRegression
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.config == "c3":
+        return bench_sv(args)
+    if args.config in ("c1", "c2") and args.x_dtype == "bf16" and args.config == "c1":
+        args.x_dtype = "fp32"  # 4 chains: the VALU tile kernel
     X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
     N, D, C = cfg["N"], cfg["D"], cfg["chains"]
     log(f"workload {args.config}: N={N} D={D} chains={C} x_dtype={args.x_dtype}")
@@ -200,6 +205,42 @@ def main():
         if fp32 and fp32["min_ess_per_s"] > 0 else None,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_sv(args):
+    """Stochastic volatility, T=3000, 128 chains: one persistent kernel launch
+    per 50 iterations (one wavefront per chain runs NUTS and the adjoint)."""
+    import torch
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.models import StochasticVolatility
+    from numpyro_amd.workloads import make_sv
+    T = args.rows or 3000
+    C = args.chains or 128
+    r, info = make_sv(T, 3)
+    model = StochasticVolatility(r, noncentred=args.noncentred)
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
+    z0 = torch.zeros((C, T + 2), dtype=torch.float32, device="cuda")
+    z0[:, 0] = float(np.log(5.0))
+    z0[:, 1 if args.noncentred else T + 1] = float(np.log(0.02))
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    mess, rhat = min_ess(out["z"])
+    ge = eng.grad_evals
+    line = {"metric": "min-ESS/sec + grad evals/sec, stochastic volatility NUTS",
+            "value": ge / wall, "unit": "grad evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic",
+            "config": {"workload": f"SV T={T}, chains={C}, "
+                                   f"{'non-centred' if args.noncentred else 'centred'}",
+                       "global_batch": C, "seq_len": T, "parallelism": "chains"},
+            "min_ess_per_s": mess / wall, "min_ess": mess, "max_rhat": rhat,
+            "mean_tree_size": float(out["num_steps"].float().mean()),
+            "mean_accept_prob": float(out["accept_prob"].float().mean()),
+            "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
+            "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
+            "hbm_bytes_per_grad_eval_per_chain": model.algorithmic_bytes(1)}
+    log(json.dumps(line))
     print(json.dumps(line), flush=True)
 
 

@@ -109,6 +109,16 @@ void nax_destroy(nax_sampler* h);
 int nax_model_logistic(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                        int32_t d, double prior_scale, int32_t x_dtype);
 
+/* Built-in potential: the stochastic-volatility model of
+ * examples/stochastic_volatility.py:58-66 (sigma ~ Exponential(50), s ~
+ * GaussianRandomWalk(sigma, T), nu ~ Exponential(0.1), r ~ StudentT(nu, 0,
+ * exp(s))), with its hand-written adjoint.  d_returns: [T] fp32; the sampler
+ * dim must be T + 2.  noncentred = 0: latent [log nu, s (T), log sigma] (the
+ * reference's model, sorted site names); 1: latent [log nu, log sigma, zeta
+ * (T)] with s = sigma * cumsum(zeta).  Sampling runs as one persistent kernel
+ * per call (one wavefront per chain). */
+int nax_model_sv(nax_sampler* h, const float* d_returns, int32_t T, int32_t noncentred);
+
 /* nax_model_logistic with the kernel choice for fp32 storage: 0 = auto
  * (matrix cores from 32 chains up), 1 = VALU tile kernel, 2 = fp32 MFMA.
  * bf16 / fp8 storage always use the matrix cores. */

@@ -1,0 +1,528 @@
+// The NUTS / HMC per-chain state machine as inline device functions, shared
+// by the tick kernel (sampler.hip) and the persistent per-model kernels.
+//
+// One wavefront owns one chain.  advance_chain() runs the chain's control flow
+// -- sample_kernel (numpyro/infer/hmc.py:459-530), build_tree
+// (numpyro/infer/hmc_util.py:1088-1180) with the iterative subtree, the
+// checkpointed U-turn checks and the uniform/biased transitions
+// (hmc_util.py:710-1085), fixed-length HMC (hmc.py:364-414) and the warmup
+// adapter (hmc_util.py:518-707) -- until the chain needs its next gradient
+// (returns true, the position is in zr) or has done `target` iterations
+// (returns false).  gi / pe_in carry the gradient and energy requested by the
+// previous call.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nax_device.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+// ---- per-chain vector slots (each [Dp] fp32) --------------------------------
+enum {
+  V_Z = 0, V_G,                    // HMCState.z, z_grad
+  V_ZL, V_RL, V_GL, V_ZR, V_RR, V_GR,  // tree edges (the subtree's outer edge lives here too)
+  V_ZP, V_GP,                      // tree proposal
+  V_RSUM,                          // tree r_sum
+  V_SZP, V_SGP, V_SRSUM,           // subtree proposal and r_sum
+  V_IMM, V_MSQRT,                  // inverse mass matrix diag, mass_matrix_sqrt diag
+  V_R0,                            // momentum drawn by init_kernel
+  V_CK                             // r_ckpts[depth], then r_sum_ckpts[depth]
+};
+
+enum {
+  PH_INIT_WAIT = 0, PH_ITER_START, PH_DOUBLING, PH_LEAF, PH_LEAF_WAIT, PH_COMBINE,
+  PH_ITER_END, PH_HMC_STEP, PH_HMC_WAIT, PH_HMC_END
+};
+
+__device__ __forceinline__ Key K(const uint32_t* k) { return Key{k[0], k[1]}; }
+__device__ __forceinline__ void setK(uint32_t* d, Key k) { d[0] = k.k0; d[1] = k.k1; }
+
+struct ChainCtx {
+  float* vb;       // chain's vector block [nvec][Dp]
+  double* wel;     // Welford [2][Dp] (mean, m2) fp64
+  int lane, D, Dp;
+  __device__ float* v(int slot) const { return vb + (size_t)slot * Dp; }
+};
+
+#define FOR_D(d) for (int d = cx.lane; d < cx.D; d += NAX_WAVE)
+
+__device__ __forceinline__ void vcopy(const ChainCtx& cx, int dst, int src) {
+  float* a = cx.v(dst);
+  const float* b = cx.v(src);
+  FOR_D(d) a[d] = b[d];
+}
+
+__device__ double kinetic(const ChainCtx& cx, int r_slot) {
+  const float* r = cx.v(r_slot);
+  const float* m = cx.v(V_IMM);
+  double acc = 0.0;
+  FOR_D(d) { double rv = r[d]; acc += (double)m[d] * rv * rv; }
+  return 0.5 * wave_sum(acc);
+}
+
+// _is_turning (hmc_util.py:710-746) with r_sum given as rs - rs_ck + r_ck when
+// rs_ck != nullptr (the iterative subtree form, hmc_util.py:961-981).
+__device__ bool is_turning(const ChainCtx& cx, const float* rl, const float* rr,
+                           const float* rs, const float* rs_ck, const float* r_ck) {
+  const float* m = cx.v(V_IMM);
+  double a = 0.0, b = 0.0;
+  FOR_D(d) {
+    float s = rs[d];
+    if (rs_ck) s = s - rs_ck[d] + r_ck[d];
+    float rlv = rl[d], rrv = rr[d];
+    float c = s - (rlv + rrv) * 0.5f;
+    a += (double)(m[d] * rlv) * c;
+    b += (double)(m[d] * rrv) * c;
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  return (a <= 0.0) || (b <= 0.0);
+}
+
+// momentum_generator (hmc.py:92-110), diagonal mass: sqrt(M) * normal(key, (D,))
+__device__ void draw_momentum(const ChainCtx& cx, int dst, Key k, bool part) {
+  float* r = cx.v(dst);
+  const float* s = cx.v(V_MSQRT);
+  FOR_D(d) r[d] = s[d] * normal_vec(k, (uint32_t)d, (uint32_t)cx.D, part);
+}
+
+// First half of velocity_verlet (hmc_util.py:289-297) on an edge, in place:
+// r <- r - eps/2 g; z <- z + eps M^-1 r; z is also written to the request.
+__device__ void leapfrog_half(const ChainCtx& cx, int zs, int rs, int gs, float eps,
+                              float* zreq) {
+  float* z = cx.v(zs);
+  float* r = cx.v(rs);
+  const float* g = cx.v(gs);
+  const float* m = cx.v(V_IMM);
+  const float h = 0.5f * eps;
+  FOR_D(d) {
+    float rh = r[d] - h * g[d];
+    float zn = z[d] + eps * (m[d] * rh);
+    r[d] = rh;
+    z[d] = zn;
+    zreq[d] = zn;
+  }
+}
+
+// Second half (hmc_util.py:298-309): g <- grad(z); r <- r - eps/2 g.
+__device__ void leapfrog_finish(const ChainCtx& cx, int rs, int gs, float eps,
+                                const float* gin) {
+  float* r = cx.v(rs);
+  float* g = cx.v(gs);
+  const float h = 0.5f * eps;
+  FOR_D(d) {
+    float gv = gin[d];
+    g[d] = gv;
+    r[d] = r[d] - h * gv;
+  }
+}
+
+// ---- warmup adaptation (hmc_util.py:60-130, 133-239, 518-707) ---------------
+
+__device__ void da_init(ChainState& S, double prox) {
+  S.da_x = 0.0; S.da_xavg = 0.0; S.da_gavg = 0.0; S.da_t = 0; S.da_prox = prox;
+}
+
+__device__ void welford_reset(const ChainCtx& cx, ChainState& S) {
+  double* mean = cx.wel;
+  double* m2 = cx.wel + cx.Dp;
+  FOR_D(d) { mean[d] = 0.0; m2[d] = 0.0; }
+  S.welford_n = 0;
+}
+
+__device__ void adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainState& S, int t,
+                             double accept_prob) {
+  const bool part = cfg.partitionable != 0;
+  Key ak = K(S.adapt_key);
+  setK(S.adapt_key, split_key(ak, 0, 2, part));  // rng_key_ss = split(...)[1] is unused
+  if (cfg.adapt_step_size) {
+    const double t0 = 10.0, kappa = 0.75, gamma = 0.05;
+    double g = cfg.target_accept - accept_prob;
+    int tt = S.da_t + 1;
+    double gavg = (1.0 - 1.0 / (tt + t0)) * S.da_gavg + g / (tt + t0);
+    double x = S.da_prox - sqrt((double)tt) / gamma * gavg;
+    double w = pow((double)tt, -kappa);
+    double xavg = (1.0 - w) * S.da_xavg + w * x;
+    S.da_t = tt; S.da_gavg = gavg; S.da_x = x; S.da_xavg = xavg;
+    double ss = (t == cfg.num_warmup - 1) ? exp(xavg) : exp(x);
+    ss = fmin(fmax(ss, 2.2250738585072014e-308), 1.7976931348623157e308);
+    S.step_size = ss;
+  }
+  const int widx = S.window_idx;
+  const bool middle = (0 < widx) && (widx < cfg.num_windows - 1);
+  if (cfg.adapt_mass_matrix && middle) {
+    double* mean = cx.wel;
+    double* m2 = cx.wel + cx.Dp;
+    const float* z = cx.v(V_Z);
+    const int n = S.welford_n + 1;
+    FOR_D(d) {
+      double x = z[d];
+      double dpre = x - mean[d];
+      double mn = mean[d] + dpre / n;
+      m2[d] += dpre * (x - mn);
+      mean[d] = mn;
+    }
+    S.welford_n = n;
+  }
+  const bool at_end = (t == cfg.window_end[widx]);
+  if (at_end) S.window_idx = widx + 1;
+  if (at_end && middle) {
+    if (cfg.adapt_mass_matrix) {
+      double* mean = cx.wel;
+      double* m2 = cx.wel + cx.Dp;
+      float* imm = cx.v(V_IMM);
+      float* msq = cx.v(V_MSQRT);
+      const double n = (double)S.welford_n;
+      FOR_D(d) {
+        double cov = m2[d] / (n - 1.0);
+        if (cfg.regularize) cov = (n / (n + 5.0)) * cov + 1e-3 * (5.0 / (n + 5.0));
+        imm[d] = (float)cov;
+        msq[d] = (float)(1.0 / sqrt(cov));
+        mean[d] = 0.0;
+        m2[d] = 0.0;
+      }
+      S.welford_n = 0;
+    }
+    if (cfg.adapt_step_size) da_init(S, log(10.0) + log(S.step_size));
+  }
+}
+
+// ---- collection (fori_collect, numpyro/util.py:384-403) ---------------------
+__device__ void collect(const EngineCfg& cfg, const Collect& col, const ChainCtx& cx,
+                        const ChainState& S, int c, int64_t it) {
+  if (col.size <= 0) return;
+  int64_t rel = it - col.start_idx;
+  if (rel < 0) return;
+  if ((rel % col.thinning) != col.thinning - 1) return;
+  int64_t slot = rel / col.thinning;
+  if (slot >= col.size) return;
+  const size_t cs = (size_t)c * col.size + slot;
+  if (col.z) {
+    float* dst = col.z + cs * (size_t)cx.D;
+    const float* z = cx.v(V_Z);
+    FOR_D(d) dst[d] = z[d];
+  }
+  if (cx.lane == 0) {
+    if (col.potential_energy) col.potential_energy[cs] = S.pe;
+    if (col.energy) col.energy[cs] = S.energy;
+    if (col.accept_prob) col.accept_prob[cs] = (float)S.accept_prob;
+    if (col.mean_accept_prob) col.mean_accept_prob[cs] = (float)S.mean_accept_prob;
+    if (col.num_steps) col.num_steps[cs] = S.num_steps;
+    if (col.diverging) col.diverging[cs] = (uint8_t)S.diverging;
+    if (col.step_size) col.step_size[cs] = (float)S.step_size;
+  }
+}
+
+__device__ __forceinline__ double nan_to_inf(double x) { return isnan(x) ? INFINITY : x; }
+__device__ __forceinline__ double clip_exp_neg(double delta) {
+  // clip(exp(-delta), None, 1)
+  return delta <= 0.0 ? 1.0 : exp(-delta);
+}
+
+// ---- the state machine -------------------------------------------------------
+__device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collect& col,
+                                              int64_t target, ChainState& S,
+                                              const ChainCtx& cx, int c, float* zr,
+                                              const float* gi, double pe_in, int& evals) {
+  const bool part = cfg.partitionable != 0;
+  const int CKR = V_CK, CKS = V_CK + cfg.max_depth_alloc;
+  bool need_grad = false;
+  while (!need_grad) {
+    switch (S.phase) {
+      case PH_INIT_WAIT: {  // init_kernel (hmc.py:338-349) after value_and_grad(z0)
+        float* g = cx.v(V_G);
+        FOR_D(d) g[d] = gi[d];
+        S.pe = pe_in;
+        S.energy = S.pe + kinetic(cx, V_R0);
+        S.phase = PH_ITER_START;
+        break;
+      }
+      case PH_ITER_START: {  // sample_kernel (hmc.py:459-490)
+        if ((int64_t)S.i >= target) return need_grad;
+        Key k0 = K(S.rng_key);
+        Key kn = split_key(k0, 0, 3, part);
+        Key kmom = split_key(k0, 1, 3, part);
+        Key ktr = split_key(k0, 2, 3, part);
+        setK(S.rng_key, kn);
+        const bool warm = S.i < cfg.num_warmup;
+        if (cfg.algo == 0) {
+          draw_momentum(cx, V_RL, kmom, part);
+          // tree = TreeInfo(z, r, z_grad, z, r, z_grad, z, pe, z_grad, energy, ...)
+          {
+            const float* z = cx.v(V_Z);
+            const float* g = cx.v(V_G);
+            float* zl = cx.v(V_ZL); float* gl = cx.v(V_GL);
+            float* zrr = cx.v(V_ZR); float* rrr = cx.v(V_RR); float* grr = cx.v(V_GR);
+            float* zp = cx.v(V_ZP); float* gp = cx.v(V_GP); float* rs = cx.v(V_RSUM);
+            const float* rl = cx.v(V_RL);
+            FOR_D(d) {
+              float zv = z[d], gv = g[d], rv = rl[d];
+              zl[d] = zv; gl[d] = gv; zrr[d] = zv; rrr[d] = rv; grr[d] = gv;
+              zp[d] = zv; gp[d] = gv; rs[d] = rv;
+            }
+          }
+          S.energy_current = S.pe + kinetic(cx, V_RL);
+          S.peP = S.pe;
+          S.energyP = S.energy_current;
+          S.depth = 0; S.weight = 0.0; S.turning = 0; S.t_diverging = 0;
+          S.sum_accept = 0.0; S.num_proposals = 0;
+          S.max_depth_cur = warm ? cfg.max_depth_warmup : cfg.max_depth_post;
+          setK(S.tree_key, ktr);
+          S.phase = PH_DOUBLING;
+        } else {
+          // _hmc_next (hmc.py:364-414): integrate edge R from (z, r, z_grad)
+          draw_momentum(cx, V_RR, kmom, part);
+          vcopy(cx, V_ZR, V_Z);
+          vcopy(cx, V_GR, V_G);
+          S.energy_current = S.pe + kinetic(cx, V_RR);
+          S.pe_edge = S.pe;
+          int ns;
+          if (cfg.fixed_num_steps > 0) ns = cfg.fixed_num_steps;
+          else ns = (int)ceil(cfg.trajectory_length / S.step_size);
+          S.hmc_num_steps = ns;
+          S.hmc_eps = (cfg.trajectory_length > 0) ? cfg.trajectory_length / ns : S.step_size;
+          S.hmc_step = 0;
+          setK(S.trans_key, ktr);
+          S.phase = PH_HMC_STEP;
+        }
+        break;
+      }
+      case PH_DOUBLING: {  // build_tree's while_loop (hmc_util.py:1156-1179)
+        if (S.depth < S.max_depth_cur && !S.turning && !S.t_diverging) {
+          Key tk = K(S.tree_key);
+          Key kn = split_key(tk, 0, 3, part);
+          Key kdir = split_key(tk, 1, 3, part);
+          Key kdbl = split_key(tk, 2, 3, part);
+          setK(S.tree_key, kn);
+          S.going_right = bernoulli(kdir, 0.5, part) ? 1 : 0;
+          setK(S.sub_key, split_key(kdbl, 0, 2, part));  // _double_tree (hmc_util.py:920)
+          setK(S.top_key, split_key(kdbl, 1, 2, part));
+          S.s_num = 0;
+          S.s_turning = 0;
+          S.s_diverging = 0;
+          S.phase = PH_LEAF;
+        } else {
+          S.phase = PH_ITER_END;
+        }
+        break;
+      }
+      case PH_LEAF: {  // _iterative_build_subtree body (hmc_util.py:1003-1017)
+        Key sk = K(S.sub_key);
+        setK(S.leaf_key, split_key(sk, 1, 2, part));
+        setK(S.sub_key, split_key(sk, 0, 2, part));
+        const float eps = (float)(S.going_right ? S.step_size : -S.step_size);
+        if (S.going_right) leapfrog_half(cx, V_ZR, V_RR, V_GR, eps, zr);
+        else leapfrog_half(cx, V_ZL, V_RL, V_GL, eps, zr);
+        S.phase = PH_LEAF_WAIT;
+        need_grad = true;
+        break;
+      }
+      case PH_LEAF_WAIT: {  // _build_basetree (hmc_util.py:851-894) + combine + ckpts
+        evals++;
+        const int zs = S.going_right ? V_ZR : V_ZL;
+        const int rs = S.going_right ? V_RR : V_RL;
+        const int gs = S.going_right ? V_GR : V_GL;
+        const float eps = (float)(S.going_right ? S.step_size : -S.step_size);
+        leapfrog_finish(cx, rs, gs, eps, gi);
+        const double pe_new = pe_in;
+        const double e_new = pe_new + kinetic(cx, rs);
+        const double delta = nan_to_inf(e_new - S.energy_current);
+        const double leaf_w = -delta;
+        const int leaf_div = delta > cfg.max_delta_energy;
+        const double acc = clip_exp_neg(delta);
+        const int leaf_idx = S.s_num;
+        const float* znew = cx.v(zs);
+        const float* rnew = cx.v(rs);
+        const float* gnew = cx.v(gs);
+        float* szp = cx.v(V_SZP);
+        float* sgp = cx.v(V_SGP);
+        float* srs = cx.v(V_SRSUM);
+        if (leaf_idx == 0) {
+          FOR_D(d) { szp[d] = znew[d]; sgp[d] = gnew[d]; srs[d] = rnew[d]; }
+          S.s_peP = pe_new; S.s_energyP = e_new; S.s_weight = leaf_w;
+          S.s_diverging = leaf_div; S.s_sum_accept = acc; S.s_num = 1;
+        } else {  // _combine_tree(..., biased_transition=False) (hmc_util.py:767-848)
+          FOR_D(d) srs[d] += rnew[d];
+          const double p = expit(leaf_w - S.s_weight);
+          if (bernoulli(K(S.leaf_key), p, part)) {
+            FOR_D(d) { szp[d] = znew[d]; sgp[d] = gnew[d]; }
+            S.s_peP = pe_new; S.s_energyP = e_new;
+          }
+          S.s_weight = log_add_exp(S.s_weight, leaf_w);
+          S.s_diverging = leaf_div;
+          S.s_sum_accept += acc;
+          S.s_num += 1;
+        }
+        // _leaf_idx_to_ckpt_idxs (hmc_util.py:941-958)
+        const int idx_max = __popc((unsigned)leaf_idx >> 1);
+        const int nsub = __popc((unsigned)((~leaf_idx & (leaf_idx + 1)) - 1));
+        const int idx_min = idx_max - nsub + 1;
+        if ((leaf_idx & 1) == 0) {
+          float* ckr = cx.v(CKR + idx_max);
+          float* cks = cx.v(CKS + idx_max);
+          FOR_D(d) { ckr[d] = rnew[d]; cks[d] = srs[d]; }
+        }
+        bool turning = false;
+        for (int i = idx_max; i >= idx_min && !turning; --i)
+          turning = is_turning(cx, cx.v(CKR + i), rnew, srs, cx.v(CKS + i), cx.v(CKR + i));
+        S.s_turning = turning ? 1 : 0;
+        if (S.s_num < (1 << S.depth) && !turning && !S.s_diverging) S.phase = PH_LEAF;
+        else S.phase = PH_COMBINE;
+        break;
+      }
+      case PH_COMBINE: {  // _double_tree's _combine_tree(..., True) (hmc_util.py:936-938)
+        float* rsum = cx.v(V_RSUM);
+        const float* srs = cx.v(V_SRSUM);
+        FOR_D(d) rsum[d] += srs[d];
+        double p = 0.0;
+        if (!S.s_turning && !S.s_diverging) p = fmin(exp(fmin(S.s_weight - S.weight, 700.0)), 1.0);
+        const bool turn = S.s_turning || is_turning(cx, cx.v(V_RL), cx.v(V_RR), rsum, nullptr, nullptr);
+        if (bernoulli(K(S.top_key), p, part)) {
+          vcopy(cx, V_ZP, V_SZP);
+          vcopy(cx, V_GP, V_SGP);
+          S.peP = S.s_peP;
+          S.energyP = S.s_energyP;
+        }
+        S.depth += 1;
+        S.weight = log_add_exp(S.weight, S.s_weight);
+        S.turning = turn ? 1 : 0;
+        S.t_diverging = S.s_diverging;
+        S.sum_accept += S.s_sum_accept;
+        S.num_proposals += S.s_num;
+        S.phase = PH_DOUBLING;
+        break;
+      }
+      case PH_HMC_STEP: {
+        if (S.hmc_step < S.hmc_num_steps) {
+          leapfrog_half(cx, V_ZR, V_RR, V_GR, (float)S.hmc_eps, zr);
+          S.phase = PH_HMC_WAIT;
+          need_grad = true;
+        } else {
+          S.phase = PH_HMC_END;
+        }
+        break;
+      }
+      case PH_HMC_WAIT: {
+        evals++;
+        leapfrog_finish(cx, V_RR, V_GR, (float)S.hmc_eps, gi);
+        S.pe_edge = pe_in;
+        S.hmc_step += 1;
+        S.phase = PH_HMC_STEP;
+        break;
+      }
+      case PH_HMC_END: {  // hmc.py:398-414
+        const double e_new = S.pe_edge + kinetic(cx, V_RR);
+        const double delta = nan_to_inf(e_new - S.energy_current);
+        const double acc = clip_exp_neg(delta);
+        S.t_diverging = delta > cfg.max_delta_energy;
+        S.sum_accept = acc;
+        S.num_proposals = S.hmc_num_steps;
+        if (bernoulli(K(S.trans_key), acc, part)) {
+          vcopy(cx, V_ZP, V_ZR);
+          vcopy(cx, V_GP, V_GR);
+          S.peP = S.pe_edge;
+          S.energyP = e_new;
+        } else {
+          vcopy(cx, V_ZP, V_Z);
+          vcopy(cx, V_GP, V_G);
+          S.peP = S.pe;
+          S.energyP = S.energy_current;
+        }
+        S.phase = PH_ITER_END;
+        break;
+      }
+      case PH_ITER_END: {  // _nuts_next (hmc.py:441-455) + sample_kernel (hmc.py:492-530)
+        const double ap = (cfg.algo == 0) ? S.sum_accept / S.num_proposals : S.sum_accept;
+        S.accept_prob = ap;
+        S.num_steps = S.num_proposals;
+        S.diverging = S.t_diverging;
+        vcopy(cx, V_Z, V_ZP);
+        vcopy(cx, V_G, V_GP);
+        S.pe = S.peP;
+        S.energy = S.energyP;
+        const int it = S.i;
+        if (it < cfg.num_warmup) adapt_update(cfg, cx, S, it, ap);
+        const int itr = it + 1;
+        const int n = (it < cfg.num_warmup) ? itr : itr - cfg.num_warmup;
+        S.mean_accept_prob = S.mean_accept_prob + (ap - S.mean_accept_prob) / n;
+        S.i = itr;
+        collect(cfg, col, cx, S, c, it);
+        S.phase = PH_ITER_START;
+        break;
+      }
+      default:
+        return need_grad;
+    }
+  }
+  return need_grad;
+}
+
+__device__ __forceinline__ ChainCtx make_ctx(const EngineCfg& cfg, int c, float* vec,
+                                             double* welf) {
+  ChainCtx cx;
+  cx.lane = threadIdx.x & (NAX_WAVE - 1);
+  cx.D = cfg.D;
+  cx.Dp = cfg.Dp;
+  cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
+  cx.wel = welf + (size_t)c * 2 * cfg.Dp;
+  return cx;
+}
+
+// ---- per-chain potentials ------------------------------------------------------
+// A per-chain model M provides
+//   __device__ double value_and_grad(const float* z, float* g, int lane) const
+// run by one wavefront: U(z) (returned on every lane) and dU/dz written to g.
+// z and g may be written by other lanes of the same wave just before the call,
+// so callers fence (workgroup scope) around it.
+
+template <class M>
+__global__ void __launch_bounds__(256)
+perchain_eval_kernel(M m, const float* __restrict__ z, int ldz, int C, double* __restrict__ pe,
+                     float* __restrict__ g, int ldg) {
+  const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
+  if (c >= C) return;
+  const int lane = threadIdx.x & (NAX_WAVE - 1);
+  const double u = m.value_and_grad(z + (size_t)c * ldz, g + (size_t)c * ldg, lane);
+  if (lane == 0) pe[c] = u;
+}
+
+// The whole sampling loop of one chain in one launch: the wave alternates
+// advance_chain() and its own potential until the chain reaches `target`.
+template <class M>
+__global__ void __launch_bounds__(256)
+persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
+                  ChainState* __restrict__ states, float* __restrict__ vec,
+                  double* __restrict__ welf, float* __restrict__ zreq, float* __restrict__ gout,
+                  double* __restrict__ peout, int64_t* __restrict__ grad_evals) {
+  const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
+  if (c >= cfg.C) return;
+  const ChainCtx cx = make_ctx(cfg, c, vec, welf);
+  float* zr = zreq + (size_t)c * cfg.Dp;
+  float* gi = gout + (size_t)c * cfg.Dp;
+  ChainState S = states[c];
+  double pe = peout[c];
+  int evals = 0;
+  while (advance_chain(cfg, col, target, S, cx, c, zr, gi, pe, evals)) {
+    __threadfence_block();
+    pe = m.value_and_grad(zr, gi, cx.lane);
+    __threadfence_block();
+  }
+  if (cx.lane == 0) {
+    states[c] = S;
+    peout[c] = pe;
+    if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
+  }
+}
+
+// ---- wave scans ------------------------------------------------------------------
+__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < NAX_WAVE; o <<= 1) {
+    const double t = __shfl_up(v, o, NAX_WAVE);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+}  // namespace nax

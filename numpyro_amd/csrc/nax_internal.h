@@ -75,6 +75,14 @@ struct Potential {
   virtual double alg_bytes(int C) const { return 0; }
   virtual double alg_flops(int C) const { return 0; }
   virtual const char* name() const { return "potential"; }
+  // Models whose potential needs no data shared across chains run the whole
+  // sampling loop in one persistent launch per call.
+  virtual bool persistent() const { return false; }
+  virtual hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                                    int64_t target, ChainState* st, float* vec, double* wel,
+                                    float* zreq, float* gout, double* peout, int64_t* gevals) {
+    return hipErrorNotSupported;
+  }
   // When set, the potential records these events on its stream immediately
   // before and after its main (dominant) kernel.
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -85,6 +93,8 @@ struct Potential {
 Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
                          int x_dtype, int fuse_variant, int num_chains, hipStream_t s,
                          std::string* err);
+
+Potential* make_sv(const float* returns, int T, int noncentred, hipStream_t s, std::string* err);
 
 void set_error(const std::string& msg);
 

@@ -1,0 +1,216 @@
+// Stochastic-volatility potential and its hand-written adjoint, one wavefront
+// per chain (examples/stochastic_volatility.py:58-66):
+//   sigma ~ Exponential(50), nu ~ Exponential(0.1),
+//   s ~ GaussianRandomWalk(sigma, T), r_t ~ StudentT(nu, 0, exp(s_t)).
+// Log-densities: Exponential continuous.py:715-726, GaussianRandomWalk
+// continuous.py:1104-1112, StudentT continuous.py:3229-3238; positive sites
+// are unconstrained by ExpTransform (transforms.py:606-652, log|J| = x) as in
+// potential_energy (numpyro/infer/util.py:300-358).
+//
+// Centred form (the reference's model), z = [log nu, s_0..s_{T-1}, log sigma]
+// (sorted site names): the GRW term is a difference + sum, so its adjoint is a
+// 3-point stencil and every t is independent -- lanes take t = lane + 64 k.
+//
+// Non-centred form, z = [log nu, log sigma, zeta_0..zeta_{T-1}] with
+// s = sigma * cumsum(zeta): the adjoint of the prefix sum is a reverse scan,
+// dU/dzeta_t = zeta_t + sigma * sum_{tau >= t} dU/ds_tau.  Lane l owns the
+// contiguous segment [lL, (l+1)L), L = ceil(T/64) (about sqrt(T) for T up to
+// ~4096): pass 1 sums each segment; a wave scan turns the sums into one
+// checkpoint per segment (the prefix at its start); pass 2 walks the segment
+// from its checkpoint computing the StudentT terms and dU/ds, whose segment
+// totals a reverse wave scan turns into suffix checkpoints; pass 3 walks the
+// segment again, recomputing s_t and dU/ds_t from the checkpoints (nothing
+// per-t is stored) and forms each suffix as (segment total - running prefix).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "nax_device.h"
+#include "nax_engine.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+// digamma for x > 0: recurrence psi(x) = psi(x + 1) - 1/x up to x >= 6, then
+// the asymptotic series ln x - 1/(2x) - sum_k B_2k / (2k x^2k)
+// (Abramowitz & Stegun 6.3.5 and 6.3.18).
+__device__ __forceinline__ double digamma_pos(double x) {
+  double acc = 0.0;
+  while (x < 6.0) {
+    acc -= 1.0 / x;
+    x += 1.0;
+  }
+  const double i2 = 1.0 / (x * x);
+  const double series =
+      i2 * (1.0 / 12 - i2 * (1.0 / 120 - i2 * (1.0 / 252 - i2 * (1.0 / 240 - i2 * (1.0 / 132)))));
+  return acc + log(x) - 0.5 / x - series;
+}
+
+constexpr double kSigmaRate = 50.0;
+constexpr double kNuRate = 0.1;
+constexpr double kLogPi = 1.1447298858494002;
+constexpr double kLog2Pi = 1.8378770664093453;
+
+struct SVModel {
+  const float* r;  // returns [T]
+  int T;
+  int noncentred;
+
+  // T-dependent constant part of U and of dU/dnu for the StudentT block.
+  __device__ void studentt_consts(double nu, double& u_const, double& dnu_const) const {
+    u_const = T * (0.5 * log(nu) + 0.5 * kLogPi + lgamma(0.5 * nu) - lgamma(0.5 * (nu + 1.0)));
+    dnu_const = T * (0.5 / nu + 0.5 * digamma_pos(0.5 * nu) - 0.5 * digamma_pos(0.5 * (nu + 1.0)));
+  }
+
+  __device__ double value_and_grad(const float* z, float* g, int lane) const {
+    return noncentred ? nc(z, g, lane) : centred(z, g, lane);
+  }
+
+  __device__ double centred(const float* z, float* g, int lane) const {
+    const double u_nu = z[0], u_sig = z[T + 1];
+    const double nu = exp(u_nu), sig = exp(u_sig);
+    const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
+    const float inv_s2 = (float)(1.0 / (sig * sig));
+    double U = 0.0, S2 = 0.0, dnu = 0.0;
+    for (int t = lane; t < T; t += NAX_WAVE) {
+      const float s = z[1 + t];
+      const float dprev = s - (t > 0 ? z[t] : 0.0f);
+      const float dnext = (t + 1 < T) ? (z[2 + t] - s) : 0.0f;
+      const float rr = r[t];
+      const float y2 = rr * rr * __expf(-2.0f * s);
+      const float q = y2 / nuf;
+      const float a = log1pf(q);
+      const float b = y2 / (nuf + y2);
+      U += (double)(0.5f * dprev * dprev * inv_s2) + 0.5 * (double)nu1 * (double)a + (double)s;
+      S2 += (double)dprev * (double)dprev;
+      dnu += 0.5 * (double)a - 0.5 * (double)nu1 * (double)b / nu;
+      g[1 + t] = (dprev - dnext) * inv_s2 + 1.0f - nu1 * b;
+    }
+    U = wave_sum(U);
+    S2 = wave_sum(S2);
+    dnu = wave_sum(dnu);
+    double uc, dnc;
+    studentt_consts(nu, uc, dnc);
+    U += uc + T * (u_sig + 0.5 * kLog2Pi)             // GRW normaliser
+         - log(kSigmaRate) + kSigmaRate * sig - u_sig    // Exponential(50) + log|J|
+         - log(kNuRate) + kNuRate * nu - u_nu;           // Exponential(0.1) + log|J|
+    if (lane == 0) {
+      g[0] = (float)(nu * (dnu + dnc + kNuRate) - 1.0);
+      g[T + 1] = (float)(-S2 / (sig * sig) + T + kSigmaRate * sig - 1.0);
+    }
+    return U;
+  }
+
+  __device__ double nc(const float* z, float* g, int lane) const {
+    const double u_nu = z[0], u_sig = z[1];
+    const double nu = exp(u_nu), sig = exp(u_sig);
+    const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
+    const float* zeta = z + 2;
+    float* gz = g + 2;
+    const int L = (T + NAX_WAVE - 1) / NAX_WAVE;
+    const int t0 = lane * L;
+    const int t1 = min(T, t0 + L);
+    // pass 1: segment sums -> prefix checkpoints
+    double seg = 0.0;
+    for (int t = t0; t < t1; ++t) seg += (double)zeta[t];
+    const double cp = wave_inclusive_scan(seg, lane) - seg;
+    // pass 2: StudentT terms and dU/ds_t = 1 - (nu+1) y^2 / (nu + y^2)
+    double U = 0.0, dnu = 0.0, A = 0.0, Asc = 0.0, Z2 = 0.0;
+    double c = cp;
+    for (int t = t0; t < t1; ++t) {
+      const float zt = zeta[t];
+      c += (double)zt;
+      const float s = (float)(sig * c);
+      const float rr = r[t];
+      const float y2 = rr * rr * __expf(-2.0f * s);
+      const float a = log1pf(y2 / nuf);
+      const float b = y2 / (nuf + y2);
+      const float ds = 1.0f - nu1 * b;
+      U += 0.5 * (double)nu1 * (double)a + (double)s;
+      dnu += 0.5 * (double)a - 0.5 * (double)nu1 * (double)b / nu;
+      A += (double)ds;
+      Asc += (double)ds * c;
+      Z2 += (double)zt * (double)zt;
+    }
+    const double Atot = wave_sum(A);
+    const double after = Atot - wave_inclusive_scan(A, lane);  // sum over later segments
+    // pass 3: dU/dzeta_t = zeta_t + sigma * (suffix of dU/ds from t)
+    double P = 0.0;
+    c = cp;
+    for (int t = t0; t < t1; ++t) {
+      const float zt = zeta[t];
+      c += (double)zt;
+      const float s = (float)(sig * c);
+      const float rr = r[t];
+      const float y2 = rr * rr * __expf(-2.0f * s);
+      const float ds = 1.0f - nu1 * (y2 / (nuf + y2));
+      gz[t] = (float)((double)zt + sig * (after + A - P));
+      P += (double)ds;
+    }
+    U = wave_sum(U) + 0.5 * wave_sum(Z2);
+    dnu = wave_sum(dnu);
+    const double asc = wave_sum(Asc);
+    double uc, dnc;
+    studentt_consts(nu, uc, dnc);
+    U += uc + T * 0.5 * kLog2Pi                         // N(0,1) normaliser of zeta
+         - log(kSigmaRate) + kSigmaRate * sig - u_sig
+         - log(kNuRate) + kNuRate * nu - u_nu;
+    if (lane == 0) {
+      g[0] = (float)(nu * (dnu + dnc + kNuRate) - 1.0);
+      g[1] = (float)(sig * asc + kSigmaRate * sig - 1.0);
+    }
+    return U;
+  }
+};
+
+struct SVPotential : Potential {
+  SVModel m;
+  float* r = nullptr;  // owned copy of the returns
+  ~SVPotential() override { hipFree(r); }
+  int dim() const override { return m.T + 2; }
+  const char* name() const override { return "sv"; }
+  bool persistent() const override { return true; }
+  double alg_bytes(int C) const override { return (double)C * 4.0 * (2.0 * (m.T + 2) + m.T); }
+
+  hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe, float* g,
+                  int ldg) override {
+    mark_begin(s);
+    hipLaunchKernelGGL(perchain_eval_kernel<SVModel>, dim3((C + 3) / 4), dim3(256), 0, s, m, z,
+                       ldz, C, pe, g, ldg);
+    mark_end(s);
+    return hipGetLastError();
+  }
+  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                            int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
+                            float* gout, double* peout, int64_t* gevals) override {
+    mark_begin(s);
+    hipLaunchKernelGGL(persistent_kernel<SVModel>, dim3((cfg.C + 3) / 4), dim3(256), 0, s, m,
+                       cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+    mark_end(s);
+    return hipGetLastError();
+  }
+};
+
+Potential* make_sv(const float* returns, int T, int noncentred, hipStream_t s, std::string* err) {
+  if (T < 2) {
+    *err = "the stochastic-volatility model needs T >= 2";
+    return nullptr;
+  }
+  auto* p = new SVPotential();
+  hipError_t e = hipMalloc(&p->r, sizeof(float) * T);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->r, returns, sizeof(float) * T, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    *err = std::string("sv model: ") + hipGetErrorString(e);
+    delete p;
+    return nullptr;
+  }
+  p->m.r = p->r;
+  p->m.T = T;
+  p->m.noncentred = noncentred;
+  return p;
+}
+
+}  // namespace nax

@@ -88,6 +88,41 @@ def _canon(x_dtype):
     return {0: "fp32", 1: "bf16", 2: "fp8"}[_lib.X_DTYPES[x_dtype]]
 
 
+class StochasticVolatility(Model):
+    """``examples/stochastic_volatility.py:58-66``: sigma ~ Exponential(50),
+    s ~ GaussianRandomWalk(sigma, T), nu ~ Exponential(0.1),
+    r ~ StudentT(nu, 0, exp(s)).  Latent (sorted site names, positive sites
+    through ExpTransform): ``[log nu, s (T), log sigma]``; with
+    ``noncentred=True`` ``[log nu, log sigma, zeta (T)]``, s = sigma cumsum(zeta)."""
+
+    def __init__(self, returns, noncentred=False):
+        r = torch.as_tensor(returns)
+        if r.ndim != 1:
+            raise ValueError("returns must be a vector")
+        self.r = r.to(device="cuda", dtype=torch.float32).contiguous()
+        self.T = int(self.r.shape[0])
+        self.dim = self.T + 2
+        self.noncentred = bool(noncentred)
+        if self.noncentred:
+            self.site_sizes = (("nu", 1, "exp"), ("sigma", 1, "exp"), ("zeta", self.T, None))
+        else:
+            self.site_sizes = (("nu", 1, "exp"), ("s", self.T, None), ("sigma", 1, "exp"))
+
+    def attach(self, handle):
+        _lib.check(_lib.load().nax_model_sv(handle, _lib.ptr(self.r), self.T,
+                                            int(self.noncentred)), "sv model")
+
+    def postprocess(self, z):
+        out = super().postprocess(z)
+        if self.noncentred:
+            out["s"] = out["sigma"][..., None] * torch.cumsum(out["zeta"], dim=-1)
+        return out
+
+    def algorithmic_bytes(self, num_chains):
+        # per gradient: z in, dU out, returns in
+        return 4.0 * num_chains * (2 * self.dim + self.T)
+
+
 class PotentialFn(Model):
     """A torch ``potential_fn(z) -> U`` on one latent vector [D]; the engine
     evaluates it for all chains with ``torch.func.vmap`` + ``grad``.  This is a

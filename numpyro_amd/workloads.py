@@ -47,3 +47,16 @@ def make(name, device="cuda", N=None, chains=None):
     logits = X @ beta
     y = (torch.rand(N, generator=g, device=device) < torch.sigmoid(logits)).float()
     return X.contiguous(), y.contiguous(), beta, cfg
+
+
+def make_sv(T=3000, seed=3):
+    """Returns generated from the stochastic-volatility model itself:
+    sigma* ~ Exponential(50), nu* ~ Exponential(0.1), s_0 ~ N(0, sigma*),
+    s_t = s_{t-1} + sigma* z_t, r_t = exp(s_t) * StudentT(nu*).  numpy PCG64."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sigma = rng.exponential(scale=1 / 50.0)
+    nu = rng.exponential(scale=1 / 0.1)
+    s = np.cumsum(sigma * rng.normal(size=T))
+    r = np.exp(s) * rng.standard_t(max(nu, 1e-3), size=T)
+    return r.astype(np.float32), dict(sigma=sigma, nu=nu, s=s, T=T, seed=seed)

@@ -120,3 +120,40 @@ def central_fd(f, z, eps=1e-5):
         e[i] = eps
         g[i] = (f(z + e) - f(z - e)) / (2 * eps)
     return g
+
+
+# ---- stochastic volatility, non-centred: s = sigma * cumsum(zeta) ----------
+# z = [log nu, log sigma, zeta_0..zeta_{T-1}]  (sorted: "nu", "sigma", "zeta");
+# zeta ~ N(0, 1) i.i.d. gives the same law of s as the Gaussian random walk
+# of examples/stochastic_volatility.py:58-66 (s_0 ~ N(0, sigma), increments
+# N(0, sigma)).
+
+def sv_nc_potential(z, returns):
+    z = np.asarray(z, dtype=np.float64)
+    T = returns.shape[0]
+    u_nu, u_sig, zeta = z[0], z[1], z[2:2 + T]
+    nu, sig = np.exp(u_nu), np.exp(u_sig)
+    s = sig * np.cumsum(zeta)
+    lp = dist.exponential_log_prob(sig, SV_SIGMA_RATE) + u_sig
+    lp += np.sum(dist.normal_log_prob(zeta))
+    lp += dist.exponential_log_prob(nu, SV_NU_RATE) + u_nu
+    lp += np.sum(dist.student_t_log_prob(returns, nu, 0.0, np.exp(s)))
+    return -lp
+
+
+def sv_nc_potential_grad(z, returns):
+    z = np.asarray(z, dtype=np.float64)
+    T = returns.shape[0]
+    u_nu, u_sig, zeta = z[0], z[1], z[2:2 + T]
+    nu, sig = np.exp(u_nu), np.exp(u_sig)
+    csum = np.cumsum(zeta)
+    s = sig * csum
+    y2 = (returns * np.exp(-s)) ** 2
+    a = 1.0 - (nu + 1.0) * y2 / (nu + y2)           # dU/ds_t
+    suffix = np.cumsum(a[::-1])[::-1]                 # sum_{tau >= t} a_tau
+    g_zeta = zeta + sig * suffix
+    g_usig = sig * np.sum(a * csum) + SV_SIGMA_RATE * sig - 1.0
+    dnu = np.sum(0.5 * np.log1p(y2 / nu) - 0.5 * (nu + 1.0) * y2 / (nu * (nu + y2))
+                 + 0.5 / nu + 0.5 * digamma(0.5 * nu) - 0.5 * digamma(0.5 * (nu + 1.0)))
+    g_unu = nu * dnu + SV_NU_RATE * nu - 1.0
+    return sv_nc_potential(z, returns), np.concatenate([[g_unu, g_usig], g_zeta])

@@ -84,3 +84,27 @@ def test_hier_grad_fd():
             + st.halfcauchy().logpdf(tau) + z[6]
             + st.bernoulli(1 / (1 + np.exp(-X @ beta))).logpmf(y).sum())
     assert_allclose(u, -want, rtol=1e-12)
+
+
+def test_sv_noncentred_grad_fd_and_equivalence():
+    rng = np.random.RandomState(4)
+    T = 15
+    r = rng.standard_t(4, size=T) * 0.02
+    zeta = rng.normal(size=T)
+    u_nu, u_sig = np.log(6.0), np.log(0.07)
+    z = np.concatenate([[u_nu, u_sig], zeta])
+    u, g = m.sv_nc_potential_grad(z, r)
+    fd = m.central_fd(lambda zz: m.sv_nc_potential(zz, r), z)
+    assert_allclose(g, fd, rtol=1e-5, atol=1e-5)
+    # same density as the centred model up to the Jacobian of s = sigma*cumsum(zeta)
+    s = np.exp(u_sig) * np.cumsum(zeta)
+    zc = np.concatenate([[u_nu], s, [u_sig]])
+    log_jac = T * u_sig  # |ds/dzeta| = sigma^T
+    assert_allclose(u, m.sv_potential(zc, r) - log_jac, rtol=1e-12)
+
+
+def test_digamma_reference_values():
+    from scipy.special import digamma
+    # psi(1) = -gamma, psi(1/2) = -gamma - 2 ln 2 (closed forms)
+    assert_allclose(digamma(1.0), -0.5772156649015329, rtol=1e-14)
+    assert_allclose(digamma(0.5), -0.5772156649015329 - 2 * np.log(2), rtol=1e-14)
