@@ -36,7 +36,8 @@ def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True):
     eng.init(z0, keys)
     t_start = time.perf_counter()
     eng.set_progress(lambda ticks, pending: log(
-        f"  {model.x_dtype}: {ticks} gradient ticks, {pending} chains running, "
+        f"  {getattr(model, 'x_dtype', type(model).__name__)}: {ticks} ticks/launches, "
+        f"{pending} chains running, "
         f"{time.perf_counter() - t_start:.0f}s"), every_ticks=progress_ticks(model, C))
     warm_s = 0.0
     if state is not None:
@@ -99,7 +100,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=150)
-    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3"])
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4"])
     ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
@@ -122,6 +123,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.config == "c3":
         return bench_sv(args)
+    if args.config == "c4":
+        return bench_hier(args)
     if args.config in ("c1", "c2") and args.x_dtype == "bf16" and args.config == "c1":
         args.x_dtype = "fp32"  # 4 chains: the VALU tile kernel
     X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
@@ -240,6 +243,41 @@ def bench_sv(args):
             "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
             "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
             "hbm_bytes_per_grad_eval_per_chain": model.algorithmic_bytes(1)}
+    log(json.dumps(line))
+    print(json.dumps(line), flush=True)
+
+
+def bench_hier(args):
+    """1024 ragged hierarchical logistic regressions x 4 chains in one
+    persistent launch per 50 iterations (a workgroup per model)."""
+    import torch
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.models import HierarchicalLogistic
+    from numpyro_amd.workloads import make_hier
+    G = args.chains or 1024
+    Xs, ys = make_hier(G, 4)
+    model = HierarchicalLogistic(Xs, ys, chains_per_group=4)
+    C = G * 4
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
+    z0 = torch.zeros((C, model.dim), dtype=torch.float32, device="cuda")
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    mess, rhat = min_ess(out["z"][:, :, :4])
+    ge = eng.grad_evals
+    line = {"metric": "min-ESS/sec + grad evals/sec, ragged hierarchical logistic NUTS",
+            "value": ge / wall, "unit": "grad evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic",
+            "config": {"workload": f"{G} hierarchical logistic regressions, N_i U[50,5000], "
+                                   f"D_i U[4,64], 4 chains each", "global_batch": C,
+                       "seq_len": int(sum(x.shape[0] for x in Xs)), "parallelism": "chains"},
+            "min_ess_per_s_first4coords": mess / wall, "max_rhat_first4coords": rhat,
+            "mean_tree_size": float(out["num_steps"].float().mean()),
+            "mean_accept_prob": float(out["accept_prob"].float().mean()),
+            "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
+            "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
+            "algorithmic_bytes_per_sweep": model.algorithmic_bytes(C),
+            "hbm_GBps_at_grad_rate": model.algorithmic_bytes(C) * ge / C / wall / 1e9}
     log(json.dumps(line))
     print(json.dumps(line), flush=True)
 

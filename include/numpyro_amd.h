@@ -119,6 +119,21 @@ int nax_model_logistic(nax_sampler* h, const float* d_x, const float* d_y, int64
  * per call (one wavefront per chain). */
 int nax_model_sv(nax_sampler* h, const float* d_returns, int32_t T, int32_t noncentred);
 
+/* Built-in potential: `groups` independent non-centred hierarchical logistic
+ * regressions (eta ~ Normal(0,1)^D_i, mu ~ Normal(0,1), tau ~ HalfCauchy(1),
+ * beta = mu + tau eta, y ~ Bernoulli(logits = X_i beta)), chains_per_group
+ * chains each (num_chains = groups * chains_per_group, chain c belongs to
+ * group c / chains_per_group).  Ragged data: d_x holds X_i row-major at
+ * element offset d_elt_off[i] (int64 [groups]); d_y holds y_i at row offset
+ * d_row_off[i] (int64 [groups + 1], N_i = d_row_off[i+1] - d_row_off[i]);
+ * d_dims (int32 [groups]) = D_i <= 64.  Chain latent (sorted site names):
+ * [eta (D_i), mu, log tau]; the sampler dim must be max D_i + 2 and each
+ * chain uses its own D_i + 2.  Sampling runs as one persistent launch, a
+ * workgroup per model. */
+int nax_model_hier(nax_sampler* h, const float* d_x, const float* d_y, const int64_t* d_row_off,
+                   const int64_t* d_elt_off, const int32_t* d_dims, int32_t groups,
+                   int32_t chains_per_group);
+
 /* nax_model_logistic with the kernel choice for fp32 storage: 0 = auto
  * (matrix cores from 32 chains up), 1 = VALU tile kernel, 2 = fp32 MFMA.
  * bf16 / fp8 storage always use the matrix cores. */

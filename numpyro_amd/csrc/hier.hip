@@ -1,0 +1,279 @@
+// Many independent non-centred hierarchical logistic regressions of uneven
+// size, K chains per model, in one launch.
+//
+// Model i (rows N_i, columns D_i <= 64):
+//   eta ~ Normal(0, 1)^D_i, mu ~ Normal(0, 1), tau ~ HalfCauchy(1),
+//   beta = mu + tau * eta, y_n ~ Bernoulli(logits = x_n . beta)
+// (Normal continuous.py:2975-2989, HalfCauchy continuous.py:1149-1150 ->
+// Cauchy continuous.py:390-406, BernoulliLogits discrete.py:244-263; tau is
+// unconstrained by ExpTransform, transforms.py:606-652, log|J| = log tau).
+// Latent of a chain, sorted site names: [eta (D_i), mu, log tau].
+//
+// Data are ragged row-major blocks: X rows of model i at elt_off[i], labels at
+// row_off[i].  A block of K wavefronts owns model i and its K chains: one X
+// tile of 64 rows is staged in LDS once and used by all K chains (phase 1:
+// lane = row, wave = chain -> logits and residuals; phase 2: lane = column,
+// wave = chain -> X^T R), so each byte of X_i is read once per gradient tick
+// of the model, not once per chain.  In the persistent form each wave runs
+// its chain's NUTS state machine; when any chain of the block needs a
+// gradient the block evaluates all K together (chains that are already done
+// ride along and ignore the result).  Blocks take models heaviest first
+// (N_i D_i descending) to even out the tail of the launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "nax_device.h"
+#include "nax_engine.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+constexpr int kHierRows = 64;
+constexpr int kHierMaxD = 64;
+constexpr double kLog2PiHalf = 0.91893853320467274;
+constexpr double kLogPi_ = 1.1447298858494002;
+constexpr double kLog2 = 0.69314718055994531;
+
+struct HierData {
+  const float* x;
+  const float* y;
+  const int64_t* row_off;
+  const int64_t* elt_off;
+  const int* dims;
+  const int* order;  // block -> model, heaviest first
+  int K;             // chains per model
+  int Dp;            // stride of a chain's latent vector
+};
+
+struct HierShared {
+  float xs[kHierRows][kHierMaxD + 1];
+  float ys[kHierRows];
+  float beta[16][kHierMaxD];
+  float res[kHierRows][16];
+};
+
+// Block-cooperative U and dU for the K chains of model i at z[c] (c = chain
+// index of the block's wave w: c = i K + w).  Wave w ends with its own
+// chain's results: returns U on every lane of wave w, writes dU to g[c].
+__device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, float* gout,
+                                  HierShared& sh) {
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int K = hd.K;
+  const int D = hd.dims[i];
+  const int64_t N = hd.row_off[i + 1] - hd.row_off[i];
+  const float* X = hd.x + hd.elt_off[i];
+  const float* Y = hd.y + hd.row_off[i];
+  const int c = i * K + w;
+  const float* z = zreq + (size_t)c * hd.Dp;
+  float* g = gout + (size_t)c * hd.Dp;
+  const float mu = z[D];
+  const float tau = expf(z[D + 1]);
+  const float eta = lane < D ? z[lane] : 0.0f;
+  sh.beta[w][lane] = lane < D ? mu + tau * eta : 0.0f;
+  float acc = 0.0f;   // phase 2: dU/dbeta_lane for chain w
+  double ul = 0.0;    // phase 1: likelihood part of U for chain w
+  const int nthreads = 64 * K;
+  for (int64_t r0 = 0; r0 < N; r0 += kHierRows) {
+    const int rows = (int)min<int64_t>(kHierRows, N - r0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * D; e += nthreads) {
+      const int r = e / D, d = e - r * D;
+      sh.xs[r][d] = X[(r0 + r) * D + d];
+    }
+    for (int r = threadIdx.x; r < rows; r += nthreads) sh.ys[r] = Y[r0 + r];
+    __syncthreads();
+    float rv = 0.0f;
+    if (lane < rows) {
+      float l = 0.0f;
+      for (int d = 0; d < D; ++d) l = fmaf(sh.xs[lane][d], sh.beta[w][d], l);
+      const float y = sh.ys[lane];
+      float u;
+      logistic_terms_f(l, y, rv, u);
+      ul += (double)u;
+    }
+    sh.res[lane][w] = rv;
+    __syncthreads();
+    if (lane < D)
+      for (int r = 0; r < rows; ++r) acc = fmaf(sh.xs[r][lane], sh.res[r][w], acc);
+  }
+  // chain w's own gradient: dU/deta = eta + tau acc, dU/dmu = mu + sum acc,
+  // dU/dlog tau = tau eta.acc + 2 tau^2 / (1 + tau^2) - 1
+  const double sum_acc = wave_sum((double)acc);
+  const double eta_acc = wave_sum((double)eta * acc);
+  const double eta2 = wave_sum((double)eta * eta);
+  const double U_lik = wave_sum(ul);
+  if (lane < D) g[lane] = eta + tau * acc;
+  const double t2 = (double)tau * tau;
+  if (lane == 0) {
+    g[D] = (float)((double)mu + sum_acc);
+    g[D + 1] = (float)((double)tau * eta_acc + 2.0 * t2 / (1.0 + t2) - 1.0);
+  }
+  const double U = U_lik + 0.5 * eta2 + D * kLog2PiHalf          // eta ~ N(0,1)
+                   + 0.5 * (double)mu * mu + kLog2PiHalf          // mu ~ N(0,1)
+                   - (kLog2 - kLogPi_ - log1p(t2))                // HalfCauchy(1)
+                   - (double)z[D + 1];                            // log|J| of exp
+  return U;
+}
+
+__global__ void __launch_bounds__(1024)
+hier_eval_kernel(HierData hd, const float* __restrict__ zreq, double* __restrict__ pe,
+                 float* __restrict__ gout) {
+  __shared__ HierShared sh;
+  const int i = hd.order[blockIdx.x];
+  const double u = hier_group_eval(hd, i, zreq, gout, sh);
+  if ((threadIdx.x & 63) == 0) pe[i * hd.K + (threadIdx.x >> 6)] = u;
+}
+
+__global__ void __launch_bounds__(1024)
+hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
+                       ChainState* __restrict__ states, float* __restrict__ vec,
+                       double* __restrict__ welf, float* __restrict__ zreq,
+                       float* __restrict__ gout, double* __restrict__ peout,
+                       int64_t* __restrict__ grad_evals) {
+  __shared__ HierShared sh;
+  __shared__ int need_any;
+  const int i = hd.order[blockIdx.x];
+  const int w = threadIdx.x >> 6;
+  const int c = i * hd.K + w;
+  const ChainCtx cx = make_ctx(cfg, c, vec, welf);
+  float* zr = zreq + (size_t)c * cfg.Dp;
+  float* gi = gout + (size_t)c * cfg.Dp;
+  ChainState S = states[c];
+  double pe = peout[c];
+  int evals = 0;
+  bool active = true;
+  for (;;) {
+    if (threadIdx.x == 0) need_any = 0;
+    __syncthreads();
+    bool need = false;
+    if (active) {
+      need = advance_chain(cfg, col, target, S, cx, c, zr, gi, pe, evals);
+      active = need;
+    }
+    if (need && cx.lane == 0) atomicOr(&need_any, 1);
+    __threadfence_block();
+    __syncthreads();
+    if (!need_any) break;
+    const double u = hier_group_eval(hd, i, zreq, gout, sh);
+    if (need) pe = u;
+    __threadfence_block();
+  }
+  if (cx.lane == 0) {
+    states[c] = S;
+    peout[c] = pe;
+    if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
+  }
+}
+
+__global__ void hier_chain_dims(const int* dims, int K, int C, int* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) out[c] = dims[c / K] + 2;
+}
+
+struct HierPotential : Potential {
+  HierData hd;
+  int groups = 0, C = 0, maxd = 0;
+  int* order = nullptr;
+  int* dims = nullptr;
+  int* cdims = nullptr;
+  int64_t* row_off = nullptr;
+  int64_t* elt_off = nullptr;
+  double flops = 0.0, bytes = 0.0;
+
+  ~HierPotential() override {
+    hipFree(order); hipFree(dims); hipFree(cdims); hipFree(row_off); hipFree(elt_off);
+  }
+  int dim() const override { return maxd + 2; }
+  const char* name() const override { return "hier"; }
+  bool persistent() const override { return true; }
+  const int* chain_dims() const override { return cdims; }
+  double alg_bytes(int) const override { return bytes; }
+  double alg_flops(int) const override { return flops; }
+
+  hipError_t eval(hipStream_t s, const float* z, int ldz, int Cn, double* pe, float* g,
+                  int ldg) override {
+    if (Cn != C || ldz != hd.Dp || ldg != hd.Dp) return hipErrorInvalidValue;
+    mark_begin(s);
+    hipLaunchKernelGGL(hier_eval_kernel, dim3(groups), dim3(64 * hd.K), 0, s, hd, z, pe, g);
+    mark_end(s);
+    return hipGetLastError();
+  }
+  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                            int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
+                            float* gout, double* peout, int64_t* gevals) override {
+    mark_begin(s);
+    hipLaunchKernelGGL(hier_persistent_kernel, dim3(groups), dim3(64 * hd.K), 0, s, hd, cfg, col,
+                       target, st, vec, wel, zreq, gout, peout, gevals);
+    mark_end(s);
+    return hipGetLastError();
+  }
+};
+
+Potential* make_hier(const float* x, const float* y, const int64_t* d_row_off,
+                     const int64_t* d_elt_off, const int* d_dims, int groups, int K, int C,
+                     int Dp, hipStream_t s, std::string* err) {
+  if (K < 1 || K > 16) {
+    *err = "hierarchical model: chains_per_group must be in [1, 16]";
+    return nullptr;
+  }
+  std::vector<int> hd_dims(groups);
+  std::vector<int64_t> hrow(groups + 1), helt(groups);
+  if (hipMemcpy(hd_dims.data(), d_dims, sizeof(int) * groups, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hrow.data(), d_row_off, sizeof(int64_t) * (groups + 1), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(helt.data(), d_elt_off, sizeof(int64_t) * groups, hipMemcpyDeviceToHost) != hipSuccess) {
+    *err = "hierarchical model: cannot read the group tables";
+    return nullptr;
+  }
+  auto* p = new HierPotential();
+  p->groups = groups;
+  p->C = C;
+  double work = 0.0;
+  std::vector<double> w(groups);
+  for (int i = 0; i < groups; ++i) {
+    const int64_t n = hrow[i + 1] - hrow[i];
+    if (hd_dims[i] < 1 || hd_dims[i] > kHierMaxD || n < 1) {
+      *err = "hierarchical model: every group needs 1 <= D_i <= 64 and N_i >= 1";
+      delete p;
+      return nullptr;
+    }
+    p->maxd = std::max(p->maxd, hd_dims[i]);
+    w[i] = (double)n * hd_dims[i];
+    work += w[i];
+  }
+  p->flops = 4.0 * K * work;         // x.beta and X^T r per chain
+  p->bytes = 4.0 * work + 4.0 * (double)(hrow[groups] - hrow[0]);  // X once per model + labels
+  std::vector<int> ord(groups);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return w[a] > w[b]; });
+  hipError_t e = hipMalloc(&p->order, sizeof(int) * groups);
+  if (e == hipSuccess) e = hipMalloc(&p->dims, sizeof(int) * groups);
+  if (e == hipSuccess) e = hipMalloc(&p->cdims, sizeof(int) * C);
+  if (e == hipSuccess) e = hipMalloc(&p->row_off, sizeof(int64_t) * (groups + 1));
+  if (e == hipSuccess) e = hipMalloc(&p->elt_off, sizeof(int64_t) * groups);
+  if (e == hipSuccess) e = hipMemcpy(p->order, ord.data(), sizeof(int) * groups, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->dims, hd_dims.data(), sizeof(int) * groups, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->row_off, hrow.data(), sizeof(int64_t) * (groups + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->elt_off, helt.data(), sizeof(int64_t) * groups, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    *err = std::string("hierarchical model: ") + hipGetErrorString(e);
+    delete p;
+    return nullptr;
+  }
+  hipLaunchKernelGGL(hier_chain_dims, dim3((C + 255) / 256), dim3(256), 0, s, p->dims, K, C, p->cdims);
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    *err = std::string("hierarchical model: ") + hipGetErrorString(e);
+    delete p;
+    return nullptr;
+  }
+  p->hd = HierData{x, y, p->row_off, p->elt_off, p->dims, p->order, K, Dp};
+  return p;
+}
+
+}  // namespace nax

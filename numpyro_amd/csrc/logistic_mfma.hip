@@ -46,16 +46,8 @@ __device__ __forceinline__ float softplus_f(float x) {
 }
 __device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
-// log-likelihood terms of one (row, chain): r = sigmoid(l) - y and
-// u = softplus(l) - y l, from e = exp(-|l|) with the hardware exp2 / rcp /
-// log2 (each within ~1 ulp; absolute error of u ~1e-7 per row).
 __device__ __forceinline__ void logit_terms(float l, float y, float& r, float& u) {
-  const float e = __builtin_amdgcn_exp2f(-fabsf(l) * 1.44269504088896341f);
-  const float d = 1.0f + e;
-  const float inv = __builtin_amdgcn_rcpf(d);
-  const float s = l >= 0.0f ? inv : e * inv;
-  r = s - y;
-  u = fmaf(__builtin_amdgcn_logf(d), 0.693147180559945309f, fmaxf(l, 0.0f) - y * l);
+  logistic_terms_f(l, y, r, u);
 }
 
 

@@ -159,4 +159,17 @@ __device__ __forceinline__ double expit(double x) {
   return x >= 0 ? 1.0 / (1.0 + exp(-x)) : exp(x) / (1.0 + exp(x));
 }
 
+// Bernoulli-logits terms of one observation: r = sigmoid(l) - y (dU/dl) and
+// u = softplus(l) - y l (= -log Bernoulli(y | logits l),
+// numpyro/distributions/util.py:317-320), from e = exp(-|l|) with the
+// hardware exp2 / rcp / log2 (each within ~1 ulp).
+__device__ __forceinline__ void logistic_terms_f(float l, float y, float& r, float& u) {
+  const float e = __builtin_amdgcn_exp2f(-fabsf(l) * 1.44269504088896341f);
+  const float d = 1.0f + e;
+  const float inv = __builtin_amdgcn_rcpf(d);
+  const float s = l >= 0.0f ? inv : e * inv;
+  r = s - y;
+  u = fmaf(__builtin_amdgcn_logf(d), 0.693147180559945309f, fmaxf(l, 0.0f) - y * l);
+}
+
 }  // namespace nax

@@ -199,7 +199,7 @@ __device__ void collect(const EngineCfg& cfg, const Collect& col, const ChainCtx
   if (slot >= col.size) return;
   const size_t cs = (size_t)c * col.size + slot;
   if (col.z) {
-    float* dst = col.z + cs * (size_t)cx.D;
+    float* dst = col.z + cs * (size_t)cfg.D;
     const float* z = cx.v(V_Z);
     FOR_D(d) dst[d] = z[d];
   }
@@ -462,7 +462,7 @@ __device__ __forceinline__ ChainCtx make_ctx(const EngineCfg& cfg, int c, float*
                                              double* welf) {
   ChainCtx cx;
   cx.lane = threadIdx.x & (NAX_WAVE - 1);
-  cx.D = cfg.D;
+  cx.D = cfg.chain_dim ? cfg.chain_dim[c] : cfg.D;
   cx.Dp = cfg.Dp;
   cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
   cx.wel = welf + (size_t)c * 2 * cfg.Dp;

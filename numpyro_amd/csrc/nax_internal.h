@@ -23,6 +23,9 @@ struct EngineCfg {
   double target_accept, max_delta_energy;
   int num_windows;
   int window_end[kMaxWindows];
+  // Per-chain latent size (ragged models), nullptr when every chain has D.
+  // Vectors are strided by Dp; entries past a chain's own size stay zero.
+  const int* chain_dim;
 };
 
 // Per-chain scalar state (one struct per chain in device memory).  All
@@ -75,6 +78,8 @@ struct Potential {
   virtual double alg_bytes(int C) const { return 0; }
   virtual double alg_flops(int C) const { return 0; }
   virtual const char* name() const { return "potential"; }
+  // Latent size of each chain on the device ([C] int32) for ragged models.
+  virtual const int* chain_dims() const { return nullptr; }
   // Models whose potential needs no data shared across chains run the whole
   // sampling loop in one persistent launch per call.
   virtual bool persistent() const { return false; }
@@ -94,6 +99,9 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
                          int x_dtype, int fuse_variant, int num_chains, hipStream_t s,
                          std::string* err);
 
+Potential* make_hier(const float* x, const float* y, const int64_t* row_off, const int64_t* elt_off,
+                     const int* dims, int groups, int chains_per_group, int num_chains, int Dp,
+                     hipStream_t s, std::string* err);
 Potential* make_sv(const float* returns, int T, int noncentred, hipStream_t s, std::string* err);
 
 void set_error(const std::string& msg);

@@ -56,12 +56,7 @@ init_kernel(EngineCfg cfg, ChainState* __restrict__ states, float* __restrict__ 
             const uint32_t* __restrict__ keys, const float* __restrict__ imm0) {
   const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
   if (c >= cfg.C) return;
-  ChainCtx cx;
-  cx.lane = threadIdx.x & (NAX_WAVE - 1);
-  cx.D = cfg.D;
-  cx.Dp = cfg.Dp;
-  cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
-  cx.wel = welf + (size_t)c * 2 * cfg.Dp;
+  const ChainCtx cx = make_ctx(cfg, c, vec, welf);
   const bool part = cfg.partitionable != 0;
   ChainState S;
   memset(&S, 0, sizeof(S));
@@ -342,6 +337,7 @@ extern "C" int nax_model_logistic_ex(nax_sampler* h, const float* x, const float
   delete h->pot;
   h->pot = p;
   h->external = false;
+  h->cfg.chain_dim = nullptr;
   return NAX_OK;
 }
 
@@ -354,6 +350,34 @@ extern "C" int nax_model_sv(nax_sampler* h, const float* returns, int32_t T, int
   delete h->pot;
   h->pot = p;
   h->external = false;
+  h->cfg.chain_dim = nullptr;
+  return NAX_OK;
+}
+
+extern "C" int nax_model_hier(nax_sampler* h, const float* x, const float* y,
+                              const int64_t* row_off, const int64_t* elt_off, const int32_t* dims,
+                              int32_t groups, int32_t chains_per_group) {
+  if (!h || !x || !y || !row_off || !elt_off || !dims) {
+    set_error("nax_model_hier: null argument");
+    return NAX_EINVAL;
+  }
+  if (groups <= 0 || chains_per_group <= 0 || (int64_t)groups * chains_per_group != h->cfg.C) {
+    set_error("hierarchical model: num_chains must equal groups * chains_per_group");
+    return NAX_EINVAL;
+  }
+  std::string err;
+  Potential* p = make_hier(x, y, row_off, elt_off, dims, groups, chains_per_group, h->cfg.C,
+                           h->cfg.Dp, h->stream, &err);
+  if (!p) { set_error(err); return NAX_EINVAL; }
+  if (p->dim() != h->cfg.D) {
+    delete p;
+    set_error("hierarchical model: the sampler dim must be max(D_i) + 2");
+    return NAX_EINVAL;
+  }
+  delete h->pot;
+  h->pot = p;
+  h->external = false;
+  h->cfg.chain_dim = p->chain_dims();
   return NAX_OK;
 }
 

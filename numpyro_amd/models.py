@@ -123,6 +123,62 @@ class StochasticVolatility(Model):
         return 4.0 * num_chains * (2 * self.dim + self.T)
 
 
+class HierarchicalLogistic(Model):
+    """Independent non-centred hierarchical logistic regressions of uneven
+    size (N_i rows, D_i <= 64 columns), ``chains_per_group`` chains each:
+    eta ~ Normal(0,1)^D_i, mu ~ Normal(0,1), tau ~ HalfCauchy(1),
+    beta = mu + tau eta, y ~ Bernoulli(logits = X_i beta).  A chain's latent
+    (sorted site names) is ``[eta (D_i), mu, log tau]``, zero-padded to
+    ``dim = max D_i + 2``; chain c belongs to group c // chains_per_group."""
+
+    def __init__(self, Xs, ys, chains_per_group=4):
+        if len(Xs) != len(ys) or not Xs:
+            raise ValueError("Xs and ys must be equally long, non-empty lists")
+        xs = [torch.as_tensor(x, dtype=torch.float32) for x in Xs]
+        yv = [torch.as_tensor(v, dtype=torch.float32) for v in ys]
+        self.dims = [int(x.shape[1]) for x in xs]
+        self.rows = [int(x.shape[0]) for x in xs]
+        if any(d < 1 or d > 64 for d in self.dims):
+            raise ValueError("every group needs 1 <= D_i <= 64")
+        self.G = len(xs)
+        self.K = int(chains_per_group)
+        self.dim = max(self.dims) + 2
+        self.X = torch.cat([x.reshape(-1) for x in xs]).to("cuda").contiguous()
+        self.y = torch.cat(yv).to("cuda").contiguous()
+        elt = [0]
+        for n, d in zip(self.rows, self.dims):
+            elt.append(elt[-1] + n * d)
+        row = [0]
+        for n in self.rows:
+            row.append(row[-1] + n)
+        self.elt_off = torch.tensor(elt[:-1], dtype=torch.int64, device="cuda")
+        self.row_off = torch.tensor(row, dtype=torch.int64, device="cuda")
+        self.dims_t = torch.tensor(self.dims, dtype=torch.int32, device="cuda")
+        self.site_sizes = ()
+
+    def attach(self, handle):
+        _lib.check(_lib.load().nax_model_hier(
+            handle, _lib.ptr(self.X), _lib.ptr(self.y), _lib.ptr(self.row_off),
+            _lib.ptr(self.elt_off), _lib.ptr(self.dims_t), self.G, self.K), "hierarchical model")
+
+    def postprocess(self, z):
+        """{"eta": [..., max D] (zero past D_i), "mu", "tau"} per chain."""
+        C = z.shape[0]
+        dims = torch.tensor(self.dims, device=z.device).repeat_interleave(self.K)[:C]
+        idx = dims.view((C,) + (1,) * (z.ndim - 1))
+        mu = torch.gather(z, -1, idx.expand(z.shape[:-1] + (1,))).squeeze(-1)
+        ut = torch.gather(z, -1, (idx + 1).expand(z.shape[:-1] + (1,))).squeeze(-1)
+        ar = torch.arange(z.shape[-1], device=z.device)
+        eta = torch.where(ar < idx, z, torch.zeros_like(z))[..., :max(self.dims)]
+        return {"eta": eta, "mu": mu, "tau": torch.exp(ut)}
+
+    def algorithmic_bytes(self, num_chains):
+        return 4.0 * sum(n * d + n for n, d in zip(self.rows, self.dims))
+
+    def algorithmic_flops(self, num_chains):
+        return 4.0 * self.K * sum(n * d for n, d in zip(self.rows, self.dims))
+
+
 class PotentialFn(Model):
     """A torch ``potential_fn(z) -> U`` on one latent vector [D]; the engine
     evaluates it for all chains with ``torch.func.vmap`` + ``grad``.  This is a

@@ -60,3 +60,24 @@ def make_sv(T=3000, seed=3):
     s = np.cumsum(sigma * rng.normal(size=T))
     r = np.exp(s) * rng.standard_t(max(nu, 1e-3), size=T)
     return r.astype(np.float32), dict(sigma=sigma, nu=nu, s=s, T=T, seed=seed)
+
+
+def make_hier(G=1024, seed=4, nmin=50, nmax=5000, dmin=4, dmax=64):
+    """Ragged hierarchical logistic workload: for each of G groups
+    N_i ~ U{nmin..nmax}, D_i ~ U{dmin..dmax}, X_i ~ N(0,1), mu_i ~ N(0,1),
+    tau_i ~ HalfCauchy(1), eta_i ~ N(0,1)^D_i, beta_i = mu_i + tau_i eta_i,
+    y_i ~ Bernoulli(sigmoid(X_i beta_i)).  numpy PCG64."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    Xs, ys = [], []
+    for _ in range(G):
+        n = int(rng.integers(nmin, nmax + 1))
+        d = int(rng.integers(dmin, dmax + 1))
+        X = rng.normal(size=(n, d)).astype(np.float32)
+        mu = rng.normal()
+        tau = abs(rng.standard_cauchy())
+        beta = mu + tau * rng.normal(size=d)
+        y = (rng.uniform(size=n) < 1 / (1 + np.exp(-np.clip(X @ beta, -60, 60)))).astype(np.float32)
+        Xs.append(X)
+        ys.append(y)
+    return Xs, ys

@@ -1,0 +1,125 @@
+"""Ragged hierarchical logistic regressions on the GPU: potential + adjoint
+against the float64 oracle, NUTS tree sizes against the oracle, and one
+group's posterior against the same model as a torch potential_fn."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import hmc as ohmc
+from oracle import jax_random as ojr
+from oracle import models as om
+
+pytestmark = pytest.mark.gpu
+
+
+def _groups(shapes, seed):
+    rng = np.random.default_rng(seed)
+    Xs, ys = [], []
+    for n, d in shapes:
+        X = rng.normal(size=(n, d)).astype(np.float32)
+        b = rng.normal() + 0.7 * rng.normal(size=d)
+        ys.append((rng.uniform(size=n) < 1 / (1 + np.exp(-X @ b))).astype(np.float32))
+        Xs.append(X)
+    return Xs, ys
+
+
+SHAPES = [(1, 1), (63, 4), (64, 17), (65, 64), (200, 9), (1000, 33), (5000, 64)]
+
+
+@pytest.mark.parametrize("K", [1, 4])
+def test_hier_potential_grad(K):
+    from numpyro_amd import _lib
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import HierarchicalLogistic
+    Xs, ys = _groups(SHAPES, K)
+    m = HierarchicalLogistic(Xs, ys, chains_per_group=K)
+    C = len(SHAPES) * K
+    eng = Engine(m, C, "NUTS", 0)
+    rng = np.random.default_rng(5)
+    Z = np.zeros((C, m.dim), np.float32)
+    for c in range(C):
+        d = SHAPES[c // K][1]
+        Z[c, :d + 2] = 0.5 * rng.normal(size=d + 2)
+    # the group kernel reads latents with the engine's padded stride
+    zp = torch.zeros((C, eng_stride(eng)), device="cuda")
+    zp[:, :m.dim] = torch.from_numpy(Z).cuda()
+    pe = torch.empty(C, dtype=torch.float64, device="cuda")
+    g = torch.zeros_like(zp)
+    _lib.check(_lib.load().nax_potential_grad(eng.h, _lib.ptr(zp), zp.shape[1], C, _lib.ptr(pe),
+                                              _lib.ptr(g)))
+    for c in range(C):
+        i, d = c // K, SHAPES[c // K][1]
+        u, gr = om.hier_potential_grad(Z[c, :d + 2].astype(np.float64), Xs[i], ys[i])
+        np.testing.assert_allclose(pe[c].item(), u, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(g[c, :d + 2].cpu().numpy(), gr, rtol=1e-4,
+                                   atol=1e-5 * (np.abs(gr).max() + 1))
+
+
+def eng_stride(eng):
+    return (eng.D + 63) // 64 * 64
+
+
+def test_hier_nuts_tree_sizes_match_oracle():
+    from numpyro_amd.infer import hmc
+    from numpyro_amd.infer.util import fori_collect
+    from numpyro_amd.models import HierarchicalLogistic
+    shapes = [(150, 3), (90, 6)]
+    K = 2
+    Xs, ys = _groups(shapes, 9)
+    m = HierarchicalLogistic(Xs, ys, chains_per_group=K)
+    C = len(shapes) * K
+    rng = np.random.default_rng(10)
+    Z = np.zeros((C, m.dim), np.float32)
+    for c in range(C):
+        d = shapes[c // K][1]
+        Z[c, :d + 2] = rng.uniform(-1, 1, size=d + 2)
+    keys = ojr.split(ojr.PRNGKey(4), C)
+    kw = dict(step_size=0.1, adapt_step_size=False, adapt_mass_matrix=False)
+    ik, sk = hmc(m)
+    st = ik(torch.from_numpy(Z), 0, rng_key=keys, **kw)
+    gns, gz = fori_collect(0, 20, sk, st, transform=lambda s: (s.num_steps, s.z))
+    gns, gz = gns.cpu().numpy(), gz.cpu().numpy()
+    for c in range(C):
+        i, d = c // K, shapes[c // K][1]
+        oik, osk = ohmc.hmc(lambda q: om.hier_potential_grad(q, Xs[i], ys[i]))
+        ost = oik(Z[c, :d + 2].astype(np.float64), 0, rng_key=keys[c], **kw)
+        ns, zs = [], []
+        for _ in range(20):
+            ost = osk(ost)
+            ns.append(ost.num_steps)
+            zs.append(ost.z)
+        assert list(gns[c]) == ns, (c, list(gns[c]), ns)
+        np.testing.assert_allclose(gz[c, :, :d + 2], np.stack(zs), rtol=1e-3, atol=1e-3)
+        assert np.all(gz[c, :, d + 2:] == 0)
+
+
+def test_hier_group_posterior_vs_torch_potential():
+    from numpyro_amd import random
+    from numpyro_amd.diagnostics import effective_sample_size
+    from numpyro_amd.infer import MCMC, NUTS
+    from numpyro_amd.models import HierarchicalLogistic
+    shapes = [(300, 5), (2000, 40), (120, 3), (800, 12)]
+    Xs, ys = _groups(shapes, 21)
+    K = 16
+    m = HierarchicalLogistic(Xs, ys, chains_per_group=K)
+    mcmc = MCMC(NUTS(m), num_warmup=300, num_samples=300, num_chains=len(shapes) * K)
+    mcmc.run(random.PRNGKey(0), init_params=torch.zeros(len(shapes) * K, m.dim))
+    post = mcmc.get_samples(group_by_chain=True)
+    mu_h = post["mu"][:K].cpu().numpy().astype(np.float64)
+    X0 = torch.from_numpy(Xs[0]).double().cuda()
+    y0 = torch.from_numpy(ys[0]).double().cuda()
+    D = shapes[0][1]
+
+    def pot(z):
+        eta, mu, ut = z[:D], z[D], z[D + 1]
+        tau = torch.exp(ut)
+        l = X0 @ (mu + tau * eta)
+        ll = torch.sum(y0 * l - torch.nn.functional.softplus(l))
+        lp = -0.5 * torch.sum(eta ** 2) - 0.5 * mu ** 2 - torch.log1p(tau ** 2) + ut
+        return -(ll + lp)
+
+    m2 = MCMC(NUTS(potential_fn=pot), num_warmup=300, num_samples=300, num_chains=K)
+    m2.run(random.PRNGKey(1), init_params=torch.zeros(K, D + 2))
+    mu_t = m2.get_samples(group_by_chain=True)[..., D].cpu().numpy().astype(np.float64)
+    se = np.sqrt(mu_h.var() / effective_sample_size(mu_h) + mu_t.var() / effective_sample_size(mu_t))
+    assert abs(mu_h.mean() - mu_t.mean()) < 4 * se, (mu_h.mean(), mu_t.mean(), se)
