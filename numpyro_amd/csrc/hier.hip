@@ -60,11 +60,16 @@ struct HierShared {
 // Block-cooperative U and dU for the K chains of model i at z[c] (c = chain
 // index of the block's wave w: c = i K + w).  Wave w ends with its own
 // chain's results: returns U on every lane of wave w, writes dU to g[c].
+// Tiles of 64 rows are contiguous (rows * D floats) in the ragged layout; the
+// next tile is loaded into registers (all loads in flight at once) while the
+// current one is processed from LDS.
+template <int K>
 __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, float* gout,
                                   HierShared& sh) {
+  constexpr int NT = 64 * K;                       // threads
+  constexpr int EPT = kHierRows * kHierMaxD / NT;  // tile elements per thread
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int K = hd.K;
   const int D = hd.dims[i];
   const int64_t N = hd.row_off[i + 1] - hd.row_off[i];
   const float* X = hd.x + hd.elt_off[i];
@@ -78,23 +83,40 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   sh.beta[w][lane] = lane < D ? mu + tau * eta : 0.0f;
   float acc = 0.0f;   // phase 2: dU/dbeta_lane for chain w
   double ul = 0.0;    // phase 1: likelihood part of U for chain w
-  const int nthreads = 64 * K;
+  float px[EPT];
+  float py = 0.0f;
+  auto load = [&](int64_t r0) {
+    const int rows = (int)min<int64_t>(kHierRows, N - r0);
+    const int ne = rows * D;
+    const float* src = X + r0 * D;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int e = threadIdx.x + NT * j;
+      px[j] = e < ne ? src[e] : 0.0f;
+    }
+    if (threadIdx.x < rows) py = Y[r0 + threadIdx.x];
+  };
+  load(0);
   for (int64_t r0 = 0; r0 < N; r0 += kHierRows) {
     const int rows = (int)min<int64_t>(kHierRows, N - r0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < rows * D; e += nthreads) {
-      const int r = e / D, d = e - r * D;
-      sh.xs[r][d] = X[(r0 + r) * D + d];
+    __syncthreads();  // previous tile fully consumed
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int e = threadIdx.x + NT * j;
+      if (e < rows * D) {
+        const int r = e / D;
+        sh.xs[r][e - r * D] = px[j];
+      }
     }
-    for (int r = threadIdx.x; r < rows; r += nthreads) sh.ys[r] = Y[r0 + r];
+    if (threadIdx.x < rows) sh.ys[threadIdx.x] = py;
     __syncthreads();
+    if (r0 + kHierRows < N) load(r0 + kHierRows);  // in flight during the compute below
     float rv = 0.0f;
     if (lane < rows) {
       float l = 0.0f;
       for (int d = 0; d < D; ++d) l = fmaf(sh.xs[lane][d], sh.beta[w][d], l);
-      const float y = sh.ys[lane];
       float u;
-      logistic_terms_f(l, y, rv, u);
+      logistic_terms_f(l, sh.ys[lane], rv, u);
       ul += (double)u;
     }
     sh.res[lane][w] = rv;
@@ -121,16 +143,18 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   return U;
 }
 
-__global__ void __launch_bounds__(1024)
+template <int K>
+__global__ void __launch_bounds__(64 * K)
 hier_eval_kernel(HierData hd, const float* __restrict__ zreq, double* __restrict__ pe,
                  float* __restrict__ gout) {
   __shared__ HierShared sh;
   const int i = hd.order[blockIdx.x];
-  const double u = hier_group_eval(hd, i, zreq, gout, sh);
-  if ((threadIdx.x & 63) == 0) pe[i * hd.K + (threadIdx.x >> 6)] = u;
+  const double u = hier_group_eval<K>(hd, i, zreq, gout, sh);
+  if ((threadIdx.x & 63) == 0) pe[i * K + (threadIdx.x >> 6)] = u;
 }
 
-__global__ void __launch_bounds__(1024)
+template <int K>
+__global__ void __launch_bounds__(64 * K)
 hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
                        ChainState* __restrict__ states, float* __restrict__ vec,
                        double* __restrict__ welf, float* __restrict__ zreq,
@@ -140,7 +164,7 @@ hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
   __shared__ int need_any;
   const int i = hd.order[blockIdx.x];
   const int w = threadIdx.x >> 6;
-  const int c = i * hd.K + w;
+  const int c = i * K + w;
   const ChainCtx cx = make_ctx(cfg, c, vec, welf);
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
@@ -160,7 +184,7 @@ hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
     __threadfence_block();
     __syncthreads();
     if (!need_any) break;
-    const double u = hier_group_eval(hd, i, zreq, gout, sh);
+    const double u = hier_group_eval<K>(hd, i, zreq, gout, sh);
     if (need) pe = u;
     __threadfence_block();
   }
@@ -200,7 +224,12 @@ struct HierPotential : Potential {
                   int ldg) override {
     if (Cn != C || ldz != hd.Dp || ldg != hd.Dp) return hipErrorInvalidValue;
     mark_begin(s);
-    hipLaunchKernelGGL(hier_eval_kernel, dim3(groups), dim3(64 * hd.K), 0, s, hd, z, pe, g);
+    switch (hd.K) {
+#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_eval_kernel<KV>, dim3(groups), dim3(64 * KV), 0, s, hd, z, pe, g); break;
+      NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
+#undef NAX_K
+      default: return hipErrorInvalidValue;
+    }
     mark_end(s);
     return hipGetLastError();
   }
@@ -208,8 +237,12 @@ struct HierPotential : Potential {
                             int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
                             float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    hipLaunchKernelGGL(hier_persistent_kernel, dim3(groups), dim3(64 * hd.K), 0, s, hd, cfg, col,
-                       target, st, vec, wel, zreq, gout, peout, gevals);
+    switch (hd.K) {
+#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_persistent_kernel<KV>, dim3(groups), dim3(64 * KV), 0, s, hd, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals); break;
+      NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
+#undef NAX_K
+      default: return hipErrorInvalidValue;
+    }
     mark_end(s);
     return hipGetLastError();
   }
@@ -218,8 +251,8 @@ struct HierPotential : Potential {
 Potential* make_hier(const float* x, const float* y, const int64_t* d_row_off,
                      const int64_t* d_elt_off, const int* d_dims, int groups, int K, int C,
                      int Dp, hipStream_t s, std::string* err) {
-  if (K < 1 || K > 16) {
-    *err = "hierarchical model: chains_per_group must be in [1, 16]";
+  if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16) {
+    *err = "hierarchical model: chains_per_group must be 1, 2, 4, 8 or 16";
     return nullptr;
   }
   std::vector<int> hd_dims(groups);

@@ -23,7 +23,7 @@ def _groups(shapes, seed):
     return Xs, ys
 
 
-SHAPES = [(1, 1), (63, 4), (64, 17), (65, 64), (200, 9), (1000, 33), (5000, 64)]
+SHAPES = [(1, 1), (63, 4), (64, 17), (65, 64), (200, 9), (1000, 33), (5000, 64), (129, 63)]
 
 
 @pytest.mark.parametrize("K", [1, 4])
