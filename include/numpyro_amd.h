@@ -49,11 +49,18 @@ typedef struct {
   int32_t regularize_mass_matrix;
   int32_t partitionable_keys; /* jax_threefry_partitionable (jax>=0.5 default: 1) */
   int32_t num_steps;
-  int32_t reserved;
+  /* find_heuristic_step_size (hmc.py:206, 319-323): run
+   * find_reasonable_step_size (hmc_util.py:314-384) at init and at the end
+   * of every middle adaptation window (hmc_util.py:572-575, 618-623). */
+  int32_t find_heuristic_step_size;
   double step_size;
   double target_accept_prob;
   double trajectory_length;
   double max_delta_energy; /* hmc.py:188: 1000 */
+  /* dense_mass=True (hmc.py:201): full inverse mass matrix per chain, dense
+   * Welford adaptation (hmc_util.py:133-239); dim <= 1024. */
+  int32_t dense_mass;
+  int32_t reserved;
 } nax_config;
 
 /* Per-iteration collection buffers, fori_collect semantics
@@ -75,7 +82,8 @@ typedef struct {
 
 /* Host view of the per-chain state (HMCState, hmc.py:31-48, with the
  * HMCAdaptState of hmc_util.py:16-30).  Device arrays are [C, dim] fp32
- * unless stated; host scalars are [C]. */
+ * unless stated (inverse mass matrix and mass_matrix_sqrt: [C, dim, dim]
+ * under dense_mass); host scalars are [C]. */
 typedef struct {
   float* d_z;
   float* d_z_grad;
@@ -147,7 +155,8 @@ int nax_model_external(nax_sampler* h);
 
 /* Initial state: init_kernel (hmc.py:193-362) for C chains.
  * d_z0 [C, dim]; keys [C, 2] host (the per-chain rng_key of init_kernel);
- * d_inverse_mass_matrix [dim] or NULL (identity). */
+ * d_inverse_mass_matrix [dim] ([dim, dim] under dense_mass) or NULL
+ * (identity), shared by all chains. */
 int nax_init(nax_sampler* h, const float* d_z0, const uint32_t* keys,
              const float* d_inverse_mass_matrix);
 

@@ -21,9 +21,11 @@ class nax_config(C.Structure):
         ("num_warmup", C.c_int32), ("max_tree_depth_warmup", C.c_int32),
         ("max_tree_depth", C.c_int32), ("adapt_step_size", C.c_int32),
         ("adapt_mass_matrix", C.c_int32), ("regularize_mass_matrix", C.c_int32),
-        ("partitionable_keys", C.c_int32), ("num_steps", C.c_int32), ("reserved", C.c_int32),
+        ("partitionable_keys", C.c_int32), ("num_steps", C.c_int32),
+        ("find_heuristic_step_size", C.c_int32),
         ("step_size", C.c_double), ("target_accept_prob", C.c_double),
         ("trajectory_length", C.c_double), ("max_delta_energy", C.c_double),
+        ("dense_mass", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

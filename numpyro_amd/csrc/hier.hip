@@ -50,28 +50,50 @@ struct HierData {
   int Dp;            // stride of a chain's latent vector
 };
 
-struct HierShared {
-  float xs[kHierRows][kHierMaxD + 1];
-  float ys[kHierRows];
-  float beta[16][kHierMaxD];
-  float res[kHierRows][16];
+// Row stride of a staged X tile: 68 floats keeps both access patterns free
+// of LDS bank conflicts (lane = row reading 16-byte column groups, lane =
+// column reading one row) and 16-byte aligned.
+constexpr int kHierLd = kHierMaxD + 4;
+
+template <int K>
+struct __attribute__((aligned(16))) HierShared {
+  __attribute__((aligned(16))) float xs[2][kHierRows][kHierLd];  // double-buffered X tile
+  __attribute__((aligned(16))) float beta[K][kHierMaxD];
+  __attribute__((aligned(16))) float res[K][kHierRows];  // residuals sigmoid(l) - y, chain-major
+  float ys[2][kHierRows];
+};
+
+// Tiles held in registers ahead of the one being processed: enough bytes in
+// flight per workgroup to cover the memory latency when a model's workgroup
+// runs alone on its CU (the tail of a launch), within ~64 VGPRs.
+template <int K>
+struct HierPipe {
+  static constexpr int NT = 64 * K;                       // threads
+  static constexpr int EPT = kHierRows * kHierMaxD / NT;  // tile elements per thread
+  static constexpr int PF = (EPT >= 32) ? 1 : 2;
 };
 
 // Block-cooperative U and dU for the K chains of model i at z[c] (c = chain
 // index of the block's wave w: c = i K + w).  Wave w ends with its own
 // chain's results: returns U on every lane of wave w, writes dU to g[c].
-// Tiles of 64 rows are contiguous (rows * D floats) in the ragged layout; the
-// next tile is loaded into registers (all loads in flight at once) while the
-// current one is processed from LDS.
+// Tiles of 64 rows are contiguous (rows * D floats) in the ragged layout.
+// Tile t is loaded into registers PF tiles ahead, staged into LDS buffer
+// t & 1, then: phase 1 (lane = row, wave = chain) logits and residuals,
+// phase 2 (lane = column, wave = chain) X^T R.  One barrier per tile: the
+// residuals a wave writes in phase 1 are read back only by that wave (LDS
+// accesses of one wavefront complete in order), and buffer t & 1 is next
+// written (tile t + 2) after every wave has passed the barrier of tile t + 1.
 template <int K>
 __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, float* gout,
-                                  HierShared& sh) {
-  constexpr int NT = 64 * K;                       // threads
-  constexpr int EPT = kHierRows * kHierMaxD / NT;  // tile elements per thread
+                                  HierShared<K>& sh) {
+  using P = HierPipe<K>;
+  constexpr int NT = P::NT, EPT = P::EPT, PF = P::PF;
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int D = hd.dims[i];
+  const int D4 = (D + 3) & ~3;
   const int64_t N = hd.row_off[i + 1] - hd.row_off[i];
+  const int ntiles = (int)((N + kHierRows - 1) / kHierRows);
   const float* X = hd.x + hd.elt_off[i];
   const float* Y = hd.y + hd.row_off[i];
   const int c = i * K + w;
@@ -81,48 +103,93 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   const float tau = expf(z[D + 1]);
   const float eta = lane < D ? z[lane] : 0.0f;
   sh.beta[w][lane] = lane < D ? mu + tau * eta : 0.0f;
-  float acc = 0.0f;   // phase 2: dU/dbeta_lane for chain w
-  double ul = 0.0;    // phase 1: likelihood part of U for chain w
-  float px[EPT];
-  float py = 0.0f;
-  auto load = [&](int64_t r0) {
+  // columns [D, D4) of both buffers stay zero for the whole evaluation
+  for (int e = threadIdx.x; e < 2 * kHierRows * 4; e += NT) {
+    const int b = e / (kHierRows * 4), r = (e / 4) % kHierRows, cc = D + (e & 3);
+    if (cc < D4) sh.xs[b][r][cc] = 0.0f;
+  }
+  // element e = threadIdx.x + NT j of a tile sits at row e / D, column e % D
+  const int step_r = NT / D, step_c = NT % D;
+  const int r_first = threadIdx.x / D, c_first = threadIdx.x % D;
+  float px[PF][EPT];
+  float py[PF];
+  auto load = [&](int t, float* dst, float& ydst) {
+    const int64_t r0 = (int64_t)t * kHierRows;
     const int rows = (int)min<int64_t>(kHierRows, N - r0);
     const int ne = rows * D;
     const float* src = X + r0 * D;
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int e = threadIdx.x + NT * j;
-      px[j] = e < ne ? src[e] : 0.0f;
+      dst[j] = e < ne ? src[e] : 0.0f;
     }
-    if (threadIdx.x < rows) py = Y[r0 + threadIdx.x];
+    ydst = threadIdx.x < rows ? Y[r0 + threadIdx.x] : 0.0f;
   };
-  load(0);
-  for (int64_t r0 = 0; r0 < N; r0 += kHierRows) {
-    const int rows = (int)min<int64_t>(kHierRows, N - r0);
-    __syncthreads();  // previous tile fully consumed
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int e = threadIdx.x + NT * j;
-      if (e < rows * D) {
-        const int r = e / D;
-        sh.xs[r][e - r * D] = px[j];
+  for (int j = 0; j < PF; ++j)
+    if (j < ntiles) load(j, px[j], py[j]);
+  float acc = 0.0f;   // phase 2: dU/dbeta_lane for chain w
+  double ul = 0.0;    // phase 1: likelihood part of U for chain w
+  for (int t0 = 0; t0 < ntiles; t0 += PF) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int t = t0 + j;
+      if (t < ntiles) {
+        const int rows = (int)min<int64_t>(kHierRows, N - (int64_t)t * kHierRows);
+        float(*xs)[kHierLd] = sh.xs[t & 1];
+        {
+          // the whole 64-row tile is written: rows past `rows` get zeros
+          const int ne = kHierRows * D;
+          int r = r_first, cc = c_first;
+#pragma unroll
+          for (int q = 0; q < EPT; ++q) {
+            if (threadIdx.x + NT * q < ne) xs[r][cc] = px[j][q];
+            r += step_r;
+            cc += step_c;
+            if (cc >= D) { cc -= D; r += 1; }
+          }
+          if (threadIdx.x < kHierRows) sh.ys[t & 1][threadIdx.x] = py[j];
+        }
+        if (t + PF < ntiles) load(t + PF, px[j], py[j]);  // in flight during the compute below
+        __syncthreads();
+        float rv = 0.0f;
+        if (lane < rows) {
+          float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
+          const float4* xr = reinterpret_cast<const float4*>(&xs[lane][0]);
+          const float4* br = reinterpret_cast<const float4*>(&sh.beta[w][0]);
+#pragma unroll
+          for (int q = 0; q < kHierMaxD / 4; ++q) {
+            if (4 * q < D4) {
+              const float4 xv = xr[q], bv = br[q];
+              l0 = fmaf(xv.x, bv.x, l0);
+              l1 = fmaf(xv.y, bv.y, l1);
+              l2 = fmaf(xv.z, bv.z, l2);
+              l3 = fmaf(xv.w, bv.w, l3);
+            }
+          }
+          float u;
+          logistic_terms_f((l0 + l1) + (l2 + l3), sh.ys[t & 1][lane], rv, u);
+          ul += (double)u;
+        }
+        sh.res[w][lane] = rv;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < D) {
+          const int rows4 = (rows + 3) & ~3;  // residuals of rows past `rows` are 0
+          const float4* rr = reinterpret_cast<const float4*>(&sh.res[w][0]);
+          float a0 = 0.0f, a1 = 0.0f;
+          for (int q = 0; q < rows4 / 4; ++q) {
+            const float4 rv4 = rr[q];
+            a0 = fmaf(xs[4 * q][lane], rv4.x, a0);
+            a1 = fmaf(xs[4 * q + 1][lane], rv4.y, a1);
+            a0 = fmaf(xs[4 * q + 2][lane], rv4.z, a0);
+            a1 = fmaf(xs[4 * q + 3][lane], rv4.w, a1);
+          }
+          acc += a0 + a1;
+        }
       }
     }
-    if (threadIdx.x < rows) sh.ys[threadIdx.x] = py;
-    __syncthreads();
-    if (r0 + kHierRows < N) load(r0 + kHierRows);  // in flight during the compute below
-    float rv = 0.0f;
-    if (lane < rows) {
-      float l = 0.0f;
-      for (int d = 0; d < D; ++d) l = fmaf(sh.xs[lane][d], sh.beta[w][d], l);
-      float u;
-      logistic_terms_f(l, sh.ys[lane], rv, u);
-      ul += (double)u;
-    }
-    sh.res[lane][w] = rv;
-    __syncthreads();
-    if (lane < D)
-      for (int r = 0; r < rows; ++r) acc = fmaf(sh.xs[r][lane], sh.res[r][w], acc);
   }
   // chain w's own gradient: dU/deta = eta + tau acc, dU/dmu = mu + sum acc,
   // dU/dlog tau = tau eta.acc + 2 tau^2 / (1 + tau^2) - 1
@@ -147,28 +214,30 @@ template <int K>
 __global__ void __launch_bounds__(64 * K)
 hier_eval_kernel(HierData hd, const float* __restrict__ zreq, double* __restrict__ pe,
                  float* __restrict__ gout) {
-  __shared__ HierShared sh;
+  __shared__ HierShared<K> sh;
   const int i = hd.order[blockIdx.x];
   const double u = hier_group_eval<K>(hd, i, zreq, gout, sh);
   if ((threadIdx.x & 63) == 0) pe[i * K + (threadIdx.x >> 6)] = u;
 }
 
-template <int K>
-__global__ void __launch_bounds__(64 * K)
+template <int K, bool DENSE>
+__global__ void __launch_bounds__(64 * K, 2)
 hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
                        ChainState* __restrict__ states, float* __restrict__ vec,
                        double* __restrict__ welf, float* __restrict__ zreq,
                        float* __restrict__ gout, double* __restrict__ peout,
                        int64_t* __restrict__ grad_evals) {
-  __shared__ HierShared sh;
+  __shared__ HierShared<K> sh;
   __shared__ int need_any;
+  __shared__ ChainState sS[K];
   const int i = hd.order[blockIdx.x];
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int c = i * K + w;
-  const ChainCtx cx = make_ctx(cfg, c, vec, welf);
+  const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
-  ChainState S = states[c];
+  ChainState& S = sS[w];
+  copy_state(S, states[c], cx.lane);
   double pe = peout[c];
   int evals = 0;
   bool active = true;
@@ -188,8 +257,8 @@ hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
     if (need) pe = u;
     __threadfence_block();
   }
+  copy_state(states[c], S, cx.lane);
   if (cx.lane == 0) {
-    states[c] = S;
     peout[c] = pe;
     if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
   }
@@ -238,7 +307,15 @@ struct HierPotential : Potential {
                             float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
     switch (hd.K) {
-#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_persistent_kernel<KV>, dim3(groups), dim3(64 * KV), 0, s, hd, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals); break;
+#define NAX_K(KV)                                                                      \
+  case KV:                                                                             \
+    if (cfg.dense)                                                                     \
+      hipLaunchKernelGGL((hier_persistent_kernel<KV, true>), dim3(groups), dim3(64 * KV), 0, s, \
+                         hd, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);      \
+    else                                                                               \
+      hipLaunchKernelGGL((hier_persistent_kernel<KV, false>), dim3(groups), dim3(64 * KV), 0, \
+                         s, hd, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);   \
+    break;
       NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
 #undef NAX_K
       default: return hipErrorInvalidValue;

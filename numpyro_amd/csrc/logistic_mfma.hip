@@ -473,6 +473,52 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     u += (double)us;
     return Ln;
   };
+  auto fused_step256 = [&](int64_t i, const f32x16& Lc, f16x8 rp0, f16x8 rp1, f16x8& r0,
+                        f16x8& r1) {
+    const bf16x8* la = (const bf16x8*)lds[(i + 1) & 3];
+    const f16x8* lb = (const f16x8*)(lds[(i - 1) & 3] + ABYTES);
+    float yv[16];
+    load_y(i, yv);
+    f32x16 Ln;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) Ln[v] = 0.0f;
+    float us = 0.0f;
+    // chunk e: 2 GEMM1 MFMAs (k-step e) + 1 GEMM2 MFMA (j = e) + the
+    // epilogue of element e; fragments for chunk e + 1 are read from LDS one
+    // chunk ahead.  Scheduling barriers keep each chunk's VALU work next to
+    // its MFMAs so that it runs in their shadow.
+    bf16x8 af = la[l];
+    f16x8 bf = lb[l];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      bf16x8 af_n;
+      f16x8 bf_n;
+      if (e + 1 < 16) {
+        af_n = la[(e + 1) * 64 + l];
+        const int jn = e + 1, m2n = jn / DT, dtn = jn % DT;
+        bf_n = lb[(m2n * DT + dtn) * 64 + l];
+      }
+      Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zh[e], Ln, 0, 0, 0);
+      {
+        const int m2 = e / DT, dt = e % DT;
+        g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(m2 ? rp1 : rp0, bf, g[dt], 0, 0, 0);
+      }
+      Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zlo[e], Ln, 0, 0, 0);
+      float r, uu;
+      logit_terms(Lc[e], yv[e], r, uu);
+      us += uu;
+      asm volatile("" : "+v"(us));  // keeps this element's loss term in the chunk
+      if (e < 8) r0[e] = (_Float16)r;
+      else r1[e - 8] = (_Float16)r;
+      if (e + 1 < 16) {
+        af = af_n;
+        bf = bf_n;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    u += (double)us;
+    return Ln;
+  };
 
   if (n > 0) {
     load_tile(0);
@@ -492,7 +538,8 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     for (int64_t i = 1; i + 1 < n; ++i) {  // steady state, branch-free body
       load_tile(i + 2);
       f16x8 r0, r1;
-      Lc = fused_step(i, Lc, rp0, rp1, r0, r1);
+      if constexpr (DP == 256) Lc = fused_step256(i, Lc, rp0, rp1, r0, r1);
+      else Lc = fused_step(i, Lc, rp0, rp1, r0, r1);
       rp0 = r0;
       rp1 = r1;
       wait_tiles();

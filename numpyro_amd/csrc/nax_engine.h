@@ -27,12 +27,14 @@ enum {
   V_SZP, V_SGP, V_SRSUM,           // subtree proposal and r_sum
   V_IMM, V_MSQRT,                  // inverse mass matrix diag, mass_matrix_sqrt diag
   V_R0,                            // momentum drawn by init_kernel
+  V_TMP,                           // scratch vector of the dense-mass products
   V_CK                             // r_ckpts[depth], then r_sum_ckpts[depth]
 };
 
 enum {
   PH_INIT_WAIT = 0, PH_ITER_START, PH_DOUBLING, PH_LEAF, PH_LEAF_WAIT, PH_COMBINE,
-  PH_ITER_END, PH_HMC_STEP, PH_HMC_WAIT, PH_HMC_END
+  PH_ITER_END, PH_HMC_STEP, PH_HMC_WAIT, PH_HMC_END,
+  PH_INIT_DONE, PH_ITER_FINISH, PH_FS_START, PH_FS_WAIT
 };
 
 __device__ __forceinline__ Key K(const uint32_t* k) { return Key{k[0], k[1]}; }
@@ -42,10 +44,32 @@ struct ChainCtx {
   float* vb;       // chain's vector block [nvec][Dp]
   double* wel;     // Welford [2][Dp] (mean, m2) fp64
   int lane, D, Dp;
+  // dense mass matrix ([Dp][Dp] row-major blocks, nullptr when diagonal):
+  // mi[k][d] = M^-1[d][k], ms[k][d] = mass_matrix_sqrt[d][k], m2 fp64
+  float* mi;
+  float* ms;
+  double* m2;
   __device__ float* v(int slot) const { return vb + (size_t)slot * Dp; }
 };
 
 #define FOR_D(d) for (int d = cx.lane; d < cx.D; d += NAX_WAVE)
+
+// Makes this wavefront's earlier global writes visible to its other lanes.
+__device__ __forceinline__ void wave_sync_mem() { __threadfence_block(); }
+
+// (A x)[d] for a matrix stored transposed (at[k][d] = A[d][k]): lane d walks
+// column d of `at` (coalesced across lanes) against the broadcast x[k].
+__device__ __forceinline__ float dense_mv(const ChainCtx& cx, const float* at, const float* x,
+                                          int d) {
+  float a0 = 0.0f, a1 = 0.0f;
+  int k = 0;
+  for (; k + 1 < cx.D; k += 2) {
+    a0 = fmaf(at[(size_t)k * cx.Dp + d], x[k], a0);
+    a1 = fmaf(at[(size_t)(k + 1) * cx.Dp + d], x[k + 1], a1);
+  }
+  if (k < cx.D) a0 = fmaf(at[(size_t)k * cx.Dp + d], x[k], a0);
+  return a0 + a1;
+}
 
 __device__ __forceinline__ void vcopy(const ChainCtx& cx, int dst, int src) {
   float* a = cx.v(dst);
@@ -53,49 +77,82 @@ __device__ __forceinline__ void vcopy(const ChainCtx& cx, int dst, int src) {
   FOR_D(d) a[d] = b[d];
 }
 
-__device__ double kinetic(const ChainCtx& cx, int r_slot) {
+// euclidean_kinetic_energy (hmc_util.py:1183-1200): 0.5 r . (M^-1 r)
+__device__ __forceinline__ double kinetic(const ChainCtx& cx, int r_slot) {
   const float* r = cx.v(r_slot);
-  const float* m = cx.v(V_IMM);
   double acc = 0.0;
-  FOR_D(d) { double rv = r[d]; acc += (double)m[d] * rv * rv; }
+  if (cx.mi) {
+    wave_sync_mem();
+    FOR_D(d) acc += (double)dense_mv(cx, cx.mi, r, d) * (double)r[d];
+  } else {
+    const float* m = cx.v(V_IMM);
+    FOR_D(d) { double rv = r[d]; acc += (double)m[d] * rv * rv; }
+  }
   return 0.5 * wave_sum(acc);
 }
 
 // _is_turning (hmc_util.py:710-746) with r_sum given as rs - rs_ck + r_ck when
 // rs_ck != nullptr (the iterative subtree form, hmc_util.py:961-981).
-__device__ bool is_turning(const ChainCtx& cx, const float* rl, const float* rr,
+__device__ __forceinline__ bool is_turning(const ChainCtx& cx, const float* rl, const float* rr,
                            const float* rs, const float* rs_ck, const float* r_ck) {
   const float* m = cx.v(V_IMM);
   double a = 0.0, b = 0.0;
+  if (cx.mi) wave_sync_mem();
   FOR_D(d) {
     float s = rs[d];
     if (rs_ck) s = s - rs_ck[d] + r_ck[d];
     float rlv = rl[d], rrv = rr[d];
     float c = s - (rlv + rrv) * 0.5f;
-    a += (double)(m[d] * rlv) * c;
-    b += (double)(m[d] * rrv) * c;
+    // velocities v = M^-1 r (_kinetic_grad, hmc_util.py:1203-1220)
+    const float vl = cx.mi ? dense_mv(cx, cx.mi, rl, d) : m[d] * rlv;
+    const float vr = cx.mi ? dense_mv(cx, cx.mi, rr, d) : m[d] * rrv;
+    a += (double)vl * c;
+    b += (double)vr * c;
   }
   a = wave_sum(a);
   b = wave_sum(b);
   return (a <= 0.0) || (b <= 0.0);
 }
 
-// momentum_generator (hmc.py:92-110), diagonal mass: sqrt(M) * normal(key, (D,))
-__device__ void draw_momentum(const ChainCtx& cx, int dst, Key k, bool part) {
+// momentum_generator (hmc.py:92-110), diagonal mass: s * normal(key, (D,)) with
+// s = sqrt(M) (the slot V_MSQRT); find_reasonable_step_size passes the inverse
+// mass matrix in that argument (hmc_util.py:359), hence the slot parameter.
+// Dense mass: r = mass_matrix_sqrt . eps (hmc.py:107-109).
+__device__ __forceinline__ void draw_momentum(const ChainCtx& cx, int dst, Key k, bool part,
+                              int scale_slot = V_MSQRT) {
   float* r = cx.v(dst);
-  const float* s = cx.v(V_MSQRT);
+  if (cx.mi) {
+    float* eps = cx.v(V_TMP);
+    FOR_D(d) eps[d] = normal_vec(k, (uint32_t)d, (uint32_t)cx.D, part);
+    wave_sync_mem();
+    const float* at = (scale_slot == V_IMM) ? cx.mi : cx.ms;
+    FOR_D(d) r[d] = dense_mv(cx, at, eps, d);
+    wave_sync_mem();
+    return;
+  }
+  const float* s = cx.v(scale_slot);
   FOR_D(d) r[d] = s[d] * normal_vec(k, (uint32_t)d, (uint32_t)cx.D, part);
 }
 
 // First half of velocity_verlet (hmc_util.py:289-297) on an edge, in place:
 // r <- r - eps/2 g; z <- z + eps M^-1 r; z is also written to the request.
-__device__ void leapfrog_half(const ChainCtx& cx, int zs, int rs, int gs, float eps,
+__device__ __forceinline__ void leapfrog_half(const ChainCtx& cx, int zs, int rs, int gs, float eps,
                               float* zreq) {
   float* z = cx.v(zs);
   float* r = cx.v(rs);
   const float* g = cx.v(gs);
   const float* m = cx.v(V_IMM);
   const float h = 0.5f * eps;
+  if (cx.mi) {
+    FOR_D(d) r[d] = r[d] - h * g[d];
+    wave_sync_mem();
+    FOR_D(d) {
+      const float zn = z[d] + eps * dense_mv(cx, cx.mi, r, d);
+      z[d] = zn;
+      zreq[d] = zn;
+    }
+    return;
+  }
   FOR_D(d) {
     float rh = r[d] - h * g[d];
     float zn = z[d] + eps * (m[d] * rh);
@@ -106,7 +163,7 @@ __device__ void leapfrog_half(const ChainCtx& cx, int zs, int rs, int gs, float 
 }
 
 // Second half (hmc_util.py:298-309): g <- grad(z); r <- r - eps/2 g.
-__device__ void leapfrog_finish(const ChainCtx& cx, int rs, int gs, float eps,
+__device__ __forceinline__ void leapfrog_finish(const ChainCtx& cx, int rs, int gs, float eps,
                                 const float* gin) {
   float* r = cx.v(rs);
   float* g = cx.v(gs);
@@ -131,11 +188,112 @@ __device__ void welford_reset(const ChainCtx& cx, ChainState& S) {
   S.welford_n = 0;
 }
 
-__device__ void adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainState& S, int t,
+// ---- dense mass matrix (dense_mass=True) --------------------------------------
+// welford_covariance update, dense (hmc_util.py:171-196): with
+// dpre = x - mean, mean += dpre / n, dpost = x - mean: m2 += outer(dpost, dpre).
+__device__ void welford_dense_update(const ChainCtx& cx, const float* z, int n) {
+  double* mean = cx.wel;
+  double* dpost = cx.wel + cx.Dp;
+  double* dpre = cx.wel + 2 * cx.Dp;
+  FOR_D(d) {
+    const double x = z[d];
+    const double a = x - mean[d];
+    const double mn = mean[d] + a / n;
+    mean[d] = mn;
+    dpre[d] = a;
+    dpost[d] = x - mn;
+  }
+  wave_sync_mem();
+  FOR_D(j) {
+    const double pj = dpre[j];
+    for (int i = 0; i < cx.D; ++i) cx.m2[(size_t)i * cx.Dp + j] += dpost[i] * pj;
+  }
+  wave_sync_mem();
+}
+
+// From a symmetric positive-definite A (fp64, in cx.m2, upper triangle read),
+// the factors welford_covariance's final_fn (hmc_util.py:227-233) and
+// _initialize_mass_matrix (hmc_util.py:497-507) take: the upper triangular U
+// with A = U U^T (the Cholesky factor of the index-reversed matrix, reversed
+// back; tril_inv = U^T) and mass_matrix_sqrt = S = (U^T)^-1, lower
+// triangular.  U overwrites A's upper triangle, S's strict lower triangle its
+// lower one (S's diagonal is 1 / diag U); S is then written to cx.ms
+// transposed (fp32).  One wavefront, O(D^3 / 3) fp64 operations.
+__device__ void dense_factor(const ChainCtx& cx) {
+  double* A = cx.m2;
+  const int D = cx.D;
+  const size_t ld = cx.Dp;
+  const int lane = cx.lane;
+  for (int j = D - 1; j >= 0; --j) {
+    double s = 0.0;
+    for (int k = j + 1 + lane; k < D; k += NAX_WAVE) {
+      const double u = A[j * ld + k];
+      s += u * u;
+    }
+    s = wave_sum(s);
+    const double ujj = sqrt(A[j * ld + j] - s);
+    for (int i = lane; i < j; i += NAX_WAVE) {
+      double t = A[i * ld + j];
+      for (int k = j + 1; k < D; ++k) t -= A[i * ld + k] * A[j * ld + k];
+      A[i * ld + j] = t / ujj;
+    }
+    if (lane == 0) A[j * ld + j] = ujj;
+    wave_sync_mem();
+  }
+  // S = L^-1 with L[i][k] = U[k][i]: forward substitution, lane = column.
+  for (int c0 = 0; c0 < D; c0 += NAX_WAVE) {
+    const int c = c0 + lane;
+    const double dinv = c < D ? 1.0 / A[(size_t)c * ld + c] : 0.0;
+    for (int i = c0 + 1; i < D; ++i) {
+      double t = 0.0;
+      for (int k = c0; k < i; ++k) {
+        const double skc = (k > c) ? A[k * ld + c] : (k == c ? dinv : 0.0);
+        t -= A[k * ld + i] * skc;
+      }
+      if (c < D && i > c) A[i * ld + c] = t / A[i * ld + i];
+    }
+  }
+  wave_sync_mem();
+  for (int k = 0; k < D; ++k)
+    FOR_D(d) {
+      const double v = (d > k) ? A[d * ld + k] : (d == k ? 1.0 / A[k * ld + k] : 0.0);
+      cx.ms[k * ld + d] = (float)v;
+    }
+  wave_sync_mem();
+}
+
+__device__ void dense_zero(const ChainCtx& cx) {
+  for (int i = 0; i < cx.D; ++i) FOR_D(j) cx.m2[(size_t)i * cx.Dp + j] = 0.0;
+  FOR_D(d) cx.wel[d] = 0.0;
+  wave_sync_mem();
+}
+
+// Window end, dense: cov = m2 / (n - 1), Stan regularisation
+// (hmc_util.py:215-226), M^-1 = cov, factors of cov, Welford reset.
+__device__ void dense_window_end(const ChainCtx& cx, double n, bool regularize) {
+  const size_t ld = cx.Dp;
+  for (int i = 0; i < cx.D; ++i)
+    FOR_D(j) {
+      double a = cx.m2[i * ld + j] / (n - 1.0);
+      if (regularize) a = (n / (n + 5.0)) * a + (i == j ? 1e-3 * (5.0 / (n + 5.0)) : 0.0);
+      cx.m2[i * ld + j] = a;
+      cx.mi[(size_t)j * ld + i] = (float)a;
+    }
+  wave_sync_mem();
+  dense_factor(cx);
+  dense_zero(cx);
+}
+
+// warmup_adapter's update_fn (hmc_util.py:637-707).  Returns true when the
+// window that ends at t is a middle window and find_reasonable_step_size must
+// run now (its key is left in S.fs_key); the dual-averaging restart of
+// _update_at_window_end (hmc_util.py:618-623) then happens when it returns.
+__device__ __forceinline__ bool adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainState& S, int t,
                              double accept_prob) {
   const bool part = cfg.partitionable != 0;
   Key ak = K(S.adapt_key);
-  setK(S.adapt_key, split_key(ak, 0, 2, part));  // rng_key_ss = split(...)[1] is unused
+  setK(S.adapt_key, split_key(ak, 0, 2, part));
+  setK(S.fs_key, split_key(ak, 1, 2, part));  // rng_key_ss
   if (cfg.adapt_step_size) {
     const double t0 = 10.0, kappa = 0.75, gamma = 0.05;
     double g = cfg.target_accept - accept_prob;
@@ -151,7 +309,10 @@ __device__ void adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainStat
   }
   const int widx = S.window_idx;
   const bool middle = (0 < widx) && (widx < cfg.num_windows - 1);
-  if (cfg.adapt_mass_matrix && middle) {
+  if (cfg.adapt_mass_matrix && middle && cx.mi) {
+    welford_dense_update(cx, cx.v(V_Z), S.welford_n + 1);
+    S.welford_n = S.welford_n + 1;
+  } else if (cfg.adapt_mass_matrix && middle) {
     double* mean = cx.wel;
     double* m2 = cx.wel + cx.Dp;
     const float* z = cx.v(V_Z);
@@ -168,7 +329,10 @@ __device__ void adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainStat
   const bool at_end = (t == cfg.window_end[widx]);
   if (at_end) S.window_idx = widx + 1;
   if (at_end && middle) {
-    if (cfg.adapt_mass_matrix) {
+    if (cfg.adapt_mass_matrix && cx.mi) {
+      dense_window_end(cx, (double)S.welford_n, cfg.regularize != 0);
+      S.welford_n = 0;
+    } else if (cfg.adapt_mass_matrix) {
       double* mean = cx.wel;
       double* m2 = cx.wel + cx.Dp;
       float* imm = cx.v(V_IMM);
@@ -184,8 +348,20 @@ __device__ void adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainStat
       }
       S.welford_n = 0;
     }
-    if (cfg.adapt_step_size) da_init(S, log(10.0) + log(S.step_size));
+    if (cfg.adapt_step_size) {
+      if (cfg.find_heuristic) return true;
+      da_init(S, log(10.0) + log(S.step_size));
+    }
   }
+  return false;
+}
+
+__device__ __forceinline__ void fs_begin(ChainState& S, int ret) {
+  S.fs_step = S.step_size;
+  S.fs_dir = 0;
+  S.fs_last_dir = 0;
+  S.fs_ret = ret;
+  S.phase = PH_FS_START;
 }
 
 // ---- collection (fori_collect, numpyro/util.py:384-403) ---------------------
@@ -234,8 +410,46 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
         float* g = cx.v(V_G);
         FOR_D(d) g[d] = gi[d];
         S.pe = pe_in;
+        if (cfg.find_heuristic && cfg.adapt_step_size) fs_begin(S, PH_INIT_DONE);
+        else S.phase = PH_INIT_DONE;
+        break;
+      }
+      case PH_INIT_DONE: {  // rest of init_kernel: energy of the initial momentum
         S.energy = S.pe + kinetic(cx, V_R0);
         S.phase = PH_ITER_START;
+        break;
+      }
+      case PH_FS_START: {  // find_reasonable_step_size's while_loop (hmc_util.py:351-383)
+        const int dir = S.fs_dir, last = S.fs_last_dir;
+        const bool small_ok = (S.fs_step > 2.2250738585072014e-308) || (dir >= 0);
+        const bool large_ok = (S.fs_step < 1.7976931348623157e308) || (dir <= 0);
+        if (!(small_ok && large_ok && (last == 0 || dir == last))) {
+          S.step_size = S.fs_step;
+          if (S.fs_ret == PH_INIT_DONE) da_init(S, log(10.0 * S.step_size));  // hmc_util.py:576
+          else da_init(S, log(10.0) + log(S.step_size));                      // hmc_util.py:623
+          S.phase = S.fs_ret;
+          break;
+        }
+        Key fk = K(S.fs_key);
+        setK(S.fs_key, split_key(fk, 0, 2, part));
+        const Key kmom = split_key(fk, 1, 2, part);
+        S.fs_step = (dir > 0) ? 2.0 * S.fs_step : (dir < 0 ? 0.5 * S.fs_step : S.fs_step);
+        draw_momentum(cx, V_RR, kmom, part, V_IMM);
+        vcopy(cx, V_ZR, V_Z);
+        vcopy(cx, V_GR, V_G);
+        S.fs_e0 = S.pe + kinetic(cx, V_RR);
+        leapfrog_half(cx, V_ZR, V_RR, V_GR, (float)S.fs_step, zr);
+        S.phase = PH_FS_WAIT;
+        need_grad = true;
+        break;
+      }
+      case PH_FS_WAIT: {
+        evals++;
+        leapfrog_finish(cx, V_RR, V_GR, (float)S.fs_step, gi);
+        const double delta = (pe_in + kinetic(cx, V_RR)) - S.fs_e0;
+        S.fs_last_dir = S.fs_dir;
+        S.fs_dir = (-0.22314355131420976 < -delta) ? 1 : -1;  // log(0.8) < -delta (NaN: -1)
+        S.phase = PH_FS_START;
         break;
       }
       case PH_ITER_START: {  // sample_kernel (hmc.py:459-490)
@@ -442,7 +656,15 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
         S.pe = S.peP;
         S.energy = S.energyP;
         const int it = S.i;
-        if (it < cfg.num_warmup) adapt_update(cfg, cx, S, it, ap);
+        bool fs = false;
+        if (it < cfg.num_warmup) fs = adapt_update(cfg, cx, S, it, ap);
+        if (fs) fs_begin(S, PH_ITER_FINISH);
+        else S.phase = PH_ITER_FINISH;
+        break;
+      }
+      case PH_ITER_FINISH: {
+        const double ap = S.accept_prob;
+        const int it = S.i;
         const int itr = it + 1;
         const int n = (it < cfg.num_warmup) ? itr : itr - cfg.num_warmup;
         S.mean_accept_prob = S.mean_accept_prob + (ap - S.mean_accept_prob) / n;
@@ -458,6 +680,31 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
   return need_grad;
 }
 
+// Chain index of the calling wavefront (`wpb` wavefronts per block), made
+// provably wave-uniform so that per-chain base addresses sit in scalar
+// registers instead of occupying a vector register pair each.
+__device__ __forceinline__ int wave_chain(int wpb) {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * wpb + threadIdx.x / NAX_WAVE));
+}
+
+// ChainState copies by all lanes of a wave.  The kernels keep the state of
+// their chains in LDS while they advance them: as a register copy the ~350
+// bytes of wave-uniform fields would occupy vector registers throughout the
+// state machine and cap the occupancy at one wave per SIMD.
+__device__ __forceinline__ void copy_state(ChainState& dst, const ChainState& src, int lane) {
+  static_assert(sizeof(ChainState) % 4 == 0, "ChainState must be whole dwords");
+  constexpr int n = sizeof(ChainState) / 4;
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(&src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(&dst);
+  for (int i = lane; i < n; i += NAX_WAVE) d[i] = s[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DENSE = false gives the diagonal-mass kernels a context whose dense
+// pointers are compile-time null, so the dense branches fold away.
+template <bool DENSE = true>
 __device__ __forceinline__ ChainCtx make_ctx(const EngineCfg& cfg, int c, float* vec,
                                              double* welf) {
   ChainCtx cx;
@@ -465,7 +712,11 @@ __device__ __forceinline__ ChainCtx make_ctx(const EngineCfg& cfg, int c, float*
   cx.D = cfg.chain_dim ? cfg.chain_dim[c] : cfg.D;
   cx.Dp = cfg.Dp;
   cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
-  cx.wel = welf + (size_t)c * 2 * cfg.Dp;
+  cx.wel = welf + (size_t)c * 3 * cfg.Dp;
+  const size_t blk = (size_t)c * cfg.Dp * cfg.Dp;
+  cx.mi = (DENSE && cfg.dense) ? cfg.dimm_t + blk : nullptr;
+  cx.ms = (DENSE && cfg.dense) ? cfg.dmsq_t + blk : nullptr;
+  cx.m2 = (DENSE && cfg.dense) ? cfg.dm2 + blk : nullptr;
   return cx;
 }
 
@@ -489,18 +740,20 @@ perchain_eval_kernel(M m, const float* __restrict__ z, int ldz, int C, double* _
 
 // The whole sampling loop of one chain in one launch: the wave alternates
 // advance_chain() and its own potential until the chain reaches `target`.
-template <class M>
+template <class M, bool DENSE>
 __global__ void __launch_bounds__(256)
 persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
                   ChainState* __restrict__ states, float* __restrict__ vec,
                   double* __restrict__ welf, float* __restrict__ zreq, float* __restrict__ gout,
                   double* __restrict__ peout, int64_t* __restrict__ grad_evals) {
-  const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
+  __shared__ ChainState sS[4];
+  const int c = wave_chain(4);
   if (c >= cfg.C) return;
-  const ChainCtx cx = make_ctx(cfg, c, vec, welf);
+  const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
-  ChainState S = states[c];
+  ChainState& S = sS[threadIdx.x / NAX_WAVE];
+  copy_state(S, states[c], cx.lane);
   double pe = peout[c];
   int evals = 0;
   while (advance_chain(cfg, col, target, S, cx, c, zr, gi, pe, evals)) {
@@ -508,8 +761,8 @@ persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
     pe = m.value_and_grad(zr, gi, cx.lane);
     __threadfence_block();
   }
+  copy_state(states[c], S, cx.lane);
   if (cx.lane == 0) {
-    states[c] = S;
     peout[c] = pe;
     if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
   }

@@ -18,6 +18,7 @@ struct EngineCfg {
   int max_depth_warmup, max_depth_post, max_depth_alloc;
   int adapt_step_size, adapt_mass_matrix, regularize;
   int partitionable;
+  int find_heuristic;         // find_reasonable_step_size at init / window ends
   int fixed_num_steps;        // <= 0: None
   double trajectory_length;   // <= 0: None
   double target_accept, max_delta_energy;
@@ -26,6 +27,13 @@ struct EngineCfg {
   // Per-chain latent size (ragged models), nullptr when every chain has D.
   // Vectors are strided by Dp; entries past a chain's own size stay zero.
   const int* chain_dim;
+  // Dense mass matrix (dense_mass=True): per chain [Dp][Dp] blocks of the
+  // transposed inverse mass matrix, the transposed mass_matrix_sqrt and the
+  // fp64 Welford co-moment matrix (also the factorisation workspace).
+  int dense;
+  float* dimm_t;
+  float* dmsq_t;
+  double* dm2;
 };
 
 // Per-chain scalar state (one struct per chain in device memory).  All
@@ -49,6 +57,10 @@ struct ChainState {
   int32_t da_t, window_idx, welford_n, pad1;
   uint32_t adapt_key[2];
   uint32_t trans_key[2];
+  // find_reasonable_step_size loop (hmc_util.py:314-384)
+  uint32_t fs_key[2];
+  int32_t fs_dir, fs_last_dir, fs_ret, pad2;
+  double fs_step, fs_e0;
 };
 
 // Collection of per-iteration fields, fori_collect semantics

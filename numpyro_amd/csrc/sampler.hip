@@ -30,20 +30,23 @@
 namespace nax {
 
 // ---- the tick kernel: each chain runs to its next gradient request -------------
+template <bool DENSE>
 __global__ void __launch_bounds__(256)
 advance_kernel(EngineCfg cfg, Collect col, int64_t target, ChainState* __restrict__ states,
                float* __restrict__ vec, double* __restrict__ welf, float* __restrict__ zreq,
                const float* __restrict__ gout, const double* __restrict__ peout,
                int* __restrict__ pending, int64_t* __restrict__ grad_evals) {
-  const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
+  __shared__ ChainState sS[4];
+  const int c = wave_chain(4);
   if (c >= cfg.C) return;
-  const ChainCtx cx = make_ctx(cfg, c, vec, welf);
-  ChainState S = states[c];
+  const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
+  ChainState& S = sS[threadIdx.x / NAX_WAVE];
+  copy_state(S, states[c], cx.lane);
   int evals = 0;
   const bool need_grad = advance_chain(cfg, col, target, S, cx, c, zreq + (size_t)c * cfg.Dp,
                                        gout + (size_t)c * cfg.Dp, peout[c], evals);
+  copy_state(states[c], S, cx.lane);
   if (cx.lane == 0) {
-    states[c] = S;
     if (need_grad) atomicAdd(pending, 1);
     if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
   }
@@ -54,7 +57,7 @@ __global__ void __launch_bounds__(256)
 init_kernel(EngineCfg cfg, ChainState* __restrict__ states, float* __restrict__ vec,
             double* __restrict__ welf, float* __restrict__ zreq, const float* __restrict__ z0,
             const uint32_t* __restrict__ keys, const float* __restrict__ imm0) {
-  const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
+  const int c = wave_chain(4);
   if (c >= cfg.C) return;
   const ChainCtx cx = make_ctx(cfg, c, vec, welf);
   const bool part = cfg.partitionable != 0;
@@ -65,13 +68,30 @@ init_kernel(EngineCfg cfg, ChainState* __restrict__ states, float* __restrict__ 
   Key k_wa = split_key(k, 1, 3, part);
   Key k_mom = split_key(k, 2, 3, part);
   setK(S.adapt_key, split_key(k_wa, 0, 2, part));
+  setK(S.fs_key, split_key(k_wa, 1, 2, part));  // rng_key_ss of hmc_util.py:565
   setK(S.rng_key, k_hmc);
   float* imm = cx.v(V_IMM);
   float* msq = cx.v(V_MSQRT);
-  FOR_D(d) {
-    float m = imm0 ? imm0[d] : 1.0f;
-    imm[d] = m;
-    msq[d] = imm0 ? (float)(1.0 / sqrt((double)m)) : 1.0f;
+  if (cx.mi) {
+    // _initialize_mass_matrix, dense (hmc_util.py:490-507): imm0 is [D][D]
+    // (identity when absent); factors as at a window end.
+    const size_t ld = cx.Dp;
+    for (int i = 0; i < cx.D; ++i)
+      FOR_D(j) {
+        const double a = imm0 ? (double)imm0[(size_t)i * cx.D + j] : (i == j ? 1.0 : 0.0);
+        cx.m2[i * ld + j] = a;
+        cx.mi[(size_t)j * ld + i] = (float)a;
+      }
+    wave_sync_mem();
+    dense_factor(cx);
+    dense_zero(cx);
+    FOR_D(d) { imm[d] = 1.0f; msq[d] = 1.0f; }
+  } else {
+    FOR_D(d) {
+      float m = imm0 ? imm0[d] : 1.0f;
+      imm[d] = m;
+      msq[d] = imm0 ? (float)(1.0 / sqrt((double)m)) : 1.0f;
+    }
   }
   S.step_size = cfg.trajectory_length;  // overwritten by the host value below
   welford_reset(cx, S);
@@ -108,6 +128,19 @@ __global__ void state_vec_io(int C, int D, int Dp, int nvec, float* vec, int slo
   float* v = vec + ((size_t)c * nvec + slot) * Dp + d;
   if (to_vec) *v = buf[(size_t)c * D + d];
   else buf[(size_t)c * D + d] = *v;
+}
+
+// Dense mass matrices: buf [C][D][D] row-major <-> the transposed per-chain
+// blocks blk [C][Dp][Dp] (blk[c][j][i] = buf[c][i][j]).
+__global__ void dense_io(int C, int D, int Dp, float* blk, float* buf, int to_blk) {
+  const int c = blockIdx.z;
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C || i >= D || j >= D) return;
+  float* b = blk + (size_t)c * Dp * Dp + (size_t)j * Dp + i;
+  float* u = buf + ((size_t)c * D + i) * D + j;
+  if (to_blk) *b = *u;
+  else *u = *b;
 }
 
 // ---- standalone leapfrog (parity entry point) ------------------------------------
@@ -275,6 +308,12 @@ extern "C" int nax_create(const nax_config* c, nax_sampler** out) {
   e.adapt_mass_matrix = c->adapt_mass_matrix;
   e.regularize = c->regularize_mass_matrix;
   e.partitionable = c->partitionable_keys;
+  e.find_heuristic = c->find_heuristic_step_size != 0;
+  e.dense = c->dense_mass != 0;
+  if (e.dense && c->dim > 1024) {
+    set_error("dense_mass supports dim <= 1024");
+    return NAX_EINVAL;
+  }
   e.fixed_num_steps = c->num_steps;
   e.trajectory_length = c->trajectory_length;
   e.target_accept = c->target_accept_prob;
@@ -289,7 +328,15 @@ extern "C" int nax_create(const nax_config* c, nax_sampler** out) {
   NAX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   const size_t C = e.C, Dp = e.Dp;
   NAX_HIP(hipMalloc(&h->vec, C * e.nvec * Dp * sizeof(float)));
-  NAX_HIP(hipMalloc(&h->wel, C * 2 * Dp * sizeof(double)));
+  NAX_HIP(hipMalloc(&h->wel, C * 3 * Dp * sizeof(double)));
+  if (e.dense) {
+    NAX_HIP(hipMalloc(&e.dimm_t, C * Dp * Dp * sizeof(float)));
+    NAX_HIP(hipMalloc(&e.dmsq_t, C * Dp * Dp * sizeof(float)));
+    NAX_HIP(hipMalloc(&e.dm2, C * Dp * Dp * sizeof(double)));
+    NAX_HIP(hipMemsetAsync(e.dimm_t, 0, C * Dp * Dp * sizeof(float), h->stream));
+    NAX_HIP(hipMemsetAsync(e.dmsq_t, 0, C * Dp * Dp * sizeof(float), h->stream));
+    NAX_HIP(hipMemsetAsync(e.dm2, 0, C * Dp * Dp * sizeof(double), h->stream));
+  }
   NAX_HIP(hipMalloc(&h->st, C * sizeof(ChainState)));
   NAX_HIP(hipMalloc(&h->zreq, C * Dp * sizeof(float)));
   NAX_HIP(hipMalloc(&h->gout, C * Dp * sizeof(float)));
@@ -312,6 +359,7 @@ extern "C" void nax_destroy(nax_sampler* h) {
   delete h->pot;
   hipFree(h->vec); hipFree(h->wel); hipFree(h->st); hipFree(h->zreq); hipFree(h->gout);
   hipFree(h->peout); hipFree(h->ctr); hipFree(h->gevals);
+  hipFree(h->cfg.dimm_t); hipFree(h->cfg.dmsq_t); hipFree(h->cfg.dm2);
   hipHostFree(h->ctr_host); hipHostFree(h->gevals_host);
   for (auto e : h->ev) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -448,8 +496,14 @@ static Collect to_collect(const nax_collect* c) {
 static int launch_advance(nax_sampler* h, const Collect& col, int64_t target, int slot) {
   const EngineCfg& e = h->cfg;
   dim3 grid((e.C + 3) / 4);
-  hipLaunchKernelGGL(advance_kernel, grid, dim3(256), 0, h->stream, e, col, target, h->st,
-                     h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot, h->gevals);
+  if (e.dense)
+    hipLaunchKernelGGL(advance_kernel<true>, grid, dim3(256), 0, h->stream, e, col, target,
+                       h->st, h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot,
+                       h->gevals);
+  else
+    hipLaunchKernelGGL(advance_kernel<false>, grid, dim3(256), 0, h->stream, e, col, target,
+                       h->st, h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot,
+                       h->gevals);
   NAX_HIP(hipGetLastError());
   return NAX_OK;
 }
@@ -479,7 +533,18 @@ extern "C" int nax_init(nax_sampler* h, const float* z0, const uint32_t* keys,
     memset(&col, 0, sizeof(col));
     rc = launch_advance(h, col, 0, 0);
     if (rc) return rc;
-    NAX_HIP(hipStreamSynchronize(h->stream));
+    // find_reasonable_step_size keeps chains asking for gradients until its
+    // doubling / halving loop ends.
+    for (;;) {
+      NAX_HIP(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+      NAX_HIP(hipStreamSynchronize(h->stream));
+      if (h->ctr_host[0] == 0) break;
+      rc = eval_potential(h);
+      if (rc) return rc;
+      hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, h->stream, h->ctr, kBatch);
+      rc = launch_advance(h, col, 0, 0);
+      if (rc) return rc;
+    }
   }
   return NAX_OK;
 }
@@ -574,8 +639,20 @@ static int state_io(nax_sampler* h, const nax_state_view* v, bool to_dev) {
   struct { float* p; int slot; } vs[] = {{v->d_z, V_Z}, {v->d_z_grad, V_G},
                                          {v->d_inverse_mass_matrix, V_IMM},
                                          {v->d_mass_matrix_sqrt, V_MSQRT}};
+  if (e.dense) {
+    // [C, dim, dim] matrices instead of the diagonal slots
+    dim3 g2((e.D + 127) / 128, e.D, e.C);
+    if (v->d_inverse_mass_matrix)
+      hipLaunchKernelGGL(dense_io, g2, dim3(128), 0, h->stream, e.C, e.D, e.Dp, e.dimm_t,
+                         v->d_inverse_mass_matrix, to_dev ? 1 : 0);
+    if (v->d_mass_matrix_sqrt)
+      hipLaunchKernelGGL(dense_io, g2, dim3(128), 0, h->stream, e.C, e.D, e.Dp, e.dmsq_t,
+                         v->d_mass_matrix_sqrt, to_dev ? 1 : 0);
+    NAX_HIP(hipGetLastError());
+  }
   for (auto& x : vs) {
     if (!x.p) continue;
+    if (e.dense && (x.slot == V_IMM || x.slot == V_MSQRT)) continue;
     hipLaunchKernelGGL(state_vec_io, grid, dim3(128), 0, h->stream, e.C, e.D, e.Dp, e.nvec,
                        h->vec, x.slot, x.p, to_dev ? 1 : 0);
     NAX_HIP(hipGetLastError());

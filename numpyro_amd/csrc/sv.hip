@@ -185,8 +185,12 @@ struct SVPotential : Potential {
                             int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
                             float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    hipLaunchKernelGGL(persistent_kernel<SVModel>, dim3((cfg.C + 3) / 4), dim3(256), 0, s, m,
-                       cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+    if (cfg.dense)
+      hipLaunchKernelGGL((persistent_kernel<SVModel, true>), dim3((cfg.C + 3) / 4), dim3(256), 0,
+                         s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+    else
+      hipLaunchKernelGGL((persistent_kernel<SVModel, false>), dim3((cfg.C + 3) / 4), dim3(256),
+                         0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
     mark_end(s);
     return hipGetLastError();
   }

@@ -55,10 +55,12 @@ class Engine:
                  regularize_mass_matrix=True, partitionable=None):
         if algo not in ("HMC", "NUTS"):
             raise ValueError("`algo` must be one of `HMC` or `NUTS`.")
-        if dense_mass not in (False, [], None):
-            raise ValueError("dense_mass is not supported by this engine (diagonal mass only)")
-        if find_heuristic_step_size:
-            raise ValueError("find_heuristic_step_size is not supported by this engine")
+        if isinstance(dense_mass, (list, tuple)):
+            if dense_mass:
+                raise ValueError("block-structured dense_mass (a list of site groups) is not "
+                                 "supported; use dense_mass=True for a full matrix")
+            dense_mass = False
+        self.dense = bool(dense_mass)
         if not torch.cuda.is_available():
             raise RuntimeError("numpyro_amd needs a HIP GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -82,6 +84,8 @@ class Engine:
         self.partitionable = bool(part)
         cfg.partitionable_keys = int(self.partitionable)
         cfg.num_steps = int(num_steps) if num_steps is not None else 0
+        cfg.find_heuristic_step_size = int(bool(find_heuristic_step_size))
+        cfg.dense_mass = int(self.dense)
         cfg.step_size = float(step_size)
         cfg.target_accept_prob = float(target_accept_prob)
         cfg.trajectory_length = float(trajectory_length) if trajectory_length is not None else 0.0
@@ -131,9 +135,13 @@ class Engine:
         imm = None
         if inverse_mass_matrix is not None:
             imm = torch.as_tensor(inverse_mass_matrix, dtype=torch.float32).to("cuda")
-            if imm.ndim == 2:
-                imm = torch.diagonal(imm)
-            imm = imm.reshape(self.D).contiguous()
+            if self.dense:  # _initialize_mass_matrix: a vector becomes diag(v)
+                imm = torch.diag(imm) if imm.ndim == 1 else imm
+                imm = imm.reshape(self.D, self.D).contiguous()
+            else:
+                if imm.ndim == 2:
+                    imm = torch.diagonal(imm)
+                imm = imm.reshape(self.D).contiguous()
         torch.cuda.synchronize()
         _lib.check(self.lib.nax_init(self.h, _lib.ptr(z0), keys.ctypes.data_as(C.c_void_p),
                                      _lib.ptr(imm)), "nax_init")
@@ -194,6 +202,9 @@ class Engine:
         Cn, D = self.C, self.D
         t = {k: torch.empty((Cn, D), dtype=torch.float32, device="cuda")
              for k in ("z", "z_grad", "imm", "msqrt")}
+        if self.dense:
+            for k in ("imm", "msqrt"):
+                t[k] = torch.empty((Cn, D, D), dtype=torch.float32, device="cuda")
         h = {"i": np.zeros(Cn, np.int32), "pe": np.zeros(Cn), "energy": np.zeros(Cn),
              "ap": np.zeros(Cn), "map": np.zeros(Cn), "ns": np.zeros(Cn, np.int32),
              "div": np.zeros(Cn, np.int32), "ss": np.zeros(Cn),
@@ -206,8 +217,11 @@ class Engine:
             h["key"].ctypes.data)
         torch.cuda.synchronize()
         _lib.check(self.lib.nax_get_state(self.h, C.byref(v)), "nax_get_state")
+        # mass_matrix_sqrt_inv: 1 / sqrt(M) (diagonal) or tril_inv = S^-1 (dense)
+        msi = (torch.linalg.inv(t["msqrt"].double()).float() if self.dense
+               else 1.0 / t["msqrt"])
         adapt = HMCAdaptState(torch.from_numpy(h["ss"]), t["imm"], t["msqrt"],
-                              1.0 / t["msqrt"], None, None, None, None)
+                              msi, None, None, None, None)
         return HMCState(torch.from_numpy(h["i"]), t["z"], t["z_grad"],
                         torch.from_numpy(h["pe"]), torch.from_numpy(h["energy"]), None,
                         self.cfg.trajectory_length or None, torch.from_numpy(h["ns"]),
@@ -221,8 +235,9 @@ class Engine:
         zg = torch.as_tensor(st.z_grad, dtype=torch.float32).to("cuda").reshape(Cn, self.D).contiguous()
         imm = torch.as_tensor(st.adapt_state.inverse_mass_matrix, dtype=torch.float32).to("cuda")
         msq = torch.as_tensor(st.adapt_state.mass_matrix_sqrt, dtype=torch.float32).to("cuda")
-        imm = imm.expand(Cn, self.D).contiguous()
-        msq = msq.expand(Cn, self.D).contiguous()
+        shape = (Cn, self.D, self.D) if self.dense else (Cn, self.D)
+        imm = imm.expand(shape).contiguous()
+        msq = msq.expand(shape).contiguous()
         keys = np.asarray(st.rng_key if rng_key is None else rng_key, np.uint32)
         keys = np.ascontiguousarray(np.broadcast_to(keys.reshape(-1, 2), (Cn, 2))).ravel()
         arr = lambda x, dt: np.ascontiguousarray(np.broadcast_to(  # noqa: E731

@@ -2,6 +2,7 @@
 
     python -m numpyro_amd.profile_kernel --config c5 --x-dtype bf16 --reps 20
     python -m numpyro_amd.profile_kernel --copy-mib 1024     # copy-kernel calibration
+    python -m numpyro_amd.profile_kernel --config c4         # ragged hierarchical models
 
 Each rep evaluates U and dU for every chain at fixed positions (one launch
 of the main kernel plus the fixed-order reduction), timed with HIP events
@@ -19,7 +20,7 @@ import torch
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1"])
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c4"])
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "mfma"])
     ap.add_argument("--reps", type=int, default=10)
@@ -34,11 +35,22 @@ def main(argv=None):
 
     out = {}
     if args.reps > 0:
-        X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
-        C_, D = cfg["chains"], cfg["D"]
-        model = LogisticRegression(X, y, x_dtype=args.x_dtype, kernel=args.kernel)
+        if args.config == "c4":
+            from .models import HierarchicalLogistic
+            G = args.chains or 1024
+            model = HierarchicalLogistic(*workloads.make_hier(G, 4), chains_per_group=4)
+            C_, D = 4 * G, model.dim
+            cfg = {"N": sum(model.rows), "chains": C_, "D": D}
+            gen = torch.Generator(device="cuda")
+            gen.manual_seed(0)
+            z = 0.5 * torch.randn((C_, D), generator=gen, device="cuda")
+            args.x_dtype = "fp32"
+        else:
+            X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
+            C_, D = cfg["chains"], cfg["D"]
+            model = LogisticRegression(X, y, x_dtype=args.x_dtype, kernel=args.kernel)
+            z = (beta[None, :] + 1e-3 * torch.randn((C_, D), device="cuda")).contiguous()
         eng = Engine(model, C_, "NUTS", 0)
-        z = (beta[None, :] + 1e-3 * torch.randn((C_, D), device="cuda")).contiguous()
         pe = torch.empty(C_, dtype=torch.float64, device="cuda")
         g = torch.empty((C_, D), dtype=torch.float32, device="cuda")
         lib = _lib.load()
@@ -57,7 +69,8 @@ def main(argv=None):
         out.update({"config": args.config, "x_dtype": args.x_dtype, "N": cfg["N"], "D": D,
                     "chains": C_, "eval_ms_min": best,
                     "eval_ms_median": sorted(times)[len(times) // 2],
-                    "algorithmic_tflops_at_min": flops / (best * 1e-3) / 1e12})
+                    "algorithmic_tflops_at_min": flops / (best * 1e-3) / 1e12,
+                    "algorithmic_GBps_at_min": model.algorithmic_bytes(C_) / (best * 1e-3) / 1e9})
         print(f"[profile] {args.x_dtype} eval: min {best:.3f} ms, median "
               f"{out['eval_ms_median']:.3f} ms", file=sys.stderr, flush=True)
     if args.copy_mib > 0:

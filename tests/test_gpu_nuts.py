@@ -90,3 +90,127 @@ def test_hmc_chain_matches_oracle():
     assert np.array_equal(gns, ons)
     np.testing.assert_allclose(gz, oz, rtol=1e-3, atol=1e-3)
     np.testing.assert_allclose(gpe, ope, rtol=1e-5)
+
+
+def _step_sizes(run, X, y, z0, keys, num_warmup, num_samples, **kw):
+    """adapt_state.step_size after init and after every iteration, [C, 1 + S]."""
+    if run == "engine":
+        from numpyro_amd.infer import hmc
+        from numpyro_amd.models import LogisticRegression
+        init_kernel, sample_kernel = hmc(LogisticRegression(X, y), algo="NUTS")
+        st = init_kernel(torch.from_numpy(z0), num_warmup, rng_key=keys, **kw)
+        rows = [torch.as_tensor(st.adapt_state.step_size).cpu().numpy().reshape(-1)]
+        ns = []
+        for _ in range(num_warmup + num_samples):
+            st = sample_kernel(st)
+            rows.append(torch.as_tensor(st.adapt_state.step_size).cpu().numpy().reshape(-1))
+            ns.append(torch.as_tensor(st.num_steps).cpu().numpy().reshape(-1))
+        return np.stack(rows, 1).astype(np.float64), np.stack(ns, 1)
+    f = lambda q: om.logistic_potential_grad(q, X, y, 1.0)  # noqa: E731
+    ss, ns = [], []
+    for c in range(z0.shape[0]):
+        ik, sk = ohmc.hmc(f, algo="NUTS")
+        st = ik(z0[c].astype(np.float64), num_warmup, rng_key=keys[c], **kw)
+        r, n = [float(st.adapt_state.step_size)], []
+        for _ in range(num_warmup + num_samples):
+            st = sk(st)
+            r.append(float(st.adapt_state.step_size))
+            n.append(int(st.num_steps))
+        ss.append(r)
+        ns.append(n)
+    return np.array(ss), np.array(ns)
+
+
+@pytest.mark.parametrize("init_step", [1.0, 1e-3])
+def test_find_heuristic_step_size_matches_oracle(init_step):
+    # find_reasonable_step_size (hmc_util.py:314-384) runs at init and at the
+    # end of the one middle window of a 60-step warmup ([0,8], [9,53], [54,59]).
+    N, D, C = 200, 5, 3
+    X, y = _problem(N, D, 31)
+    z0 = np.random.default_rng(7).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(17), C)[c] for c in range(C)])
+    kw = dict(step_size=init_step, find_heuristic_step_size=True)
+    gss, gns = _step_sizes("engine", X, y, z0, keys, 60, 10, **kw)
+    oss, ons = _step_sizes("oracle", X, y, z0, keys, 60, 10, **kw)
+    # the init heuristic multiplies init_step by a power of two: exact
+    np.testing.assert_array_equal(gss[:, 0], oss[:, 0])
+    assert np.all(gss[:, 0] != init_step)
+    assert np.array_equal(gns, ons), (gns, ons)
+    # dual averaging amplifies float32 rounding (the oracle against itself
+    # drifts ~1e-3 by the end of warmup); a wrong doubling or halving at the
+    # window end would show as a factor of 2
+    np.testing.assert_allclose(gss[:, :26], oss[:, :26], rtol=1e-3)
+    np.testing.assert_allclose(gss, oss, rtol=2e-2)
+
+
+def _spd(D, seed):
+    a = np.random.default_rng(seed).normal(size=(D, D))
+    return (a @ a.T / D + 0.5 * np.eye(D)).astype(np.float32)
+
+
+def _run_pair(X, y, z0, keys, num_warmup, num_samples, **kw):
+    """Engine and oracle chains side by side: per-iteration z and num_steps,
+    and the final adapt_state (inverse mass matrix, mass_matrix_sqrt)."""
+    from numpyro_amd.infer import hmc
+    from numpyro_amd.models import LogisticRegression
+    init_kernel, sample_kernel = hmc(LogisticRegression(X, y), algo="NUTS")
+    st = init_kernel(torch.from_numpy(z0), num_warmup, rng_key=keys, **kw)
+    gz, gns = [], []
+    for _ in range(num_warmup + num_samples):
+        st = sample_kernel(st)
+        gz.append(torch.as_tensor(st.z).cpu().numpy())
+        gns.append(torch.as_tensor(st.num_steps).cpu().numpy().reshape(-1))
+    g = dict(z=np.stack(gz, 1), ns=np.stack(gns, 1),
+             imm=torch.as_tensor(st.adapt_state.inverse_mass_matrix).cpu().numpy(),
+             msq=torch.as_tensor(st.adapt_state.mass_matrix_sqrt).cpu().numpy())
+    f = lambda q: om.logistic_potential_grad(q, X, y, 1.0)  # noqa: E731
+    o = dict(z=[], ns=[], imm=[], msq=[])
+    for c in range(z0.shape[0]):
+        ik, sk = ohmc.hmc(f, algo="NUTS")
+        s = ik(z0[c].astype(np.float64), num_warmup, rng_key=keys[c], **kw)
+        zz, nn = [], []
+        for _ in range(num_warmup + num_samples):
+            s = sk(s)
+            zz.append(np.copy(s.z))
+            nn.append(int(s.num_steps))
+        o["z"].append(zz)
+        o["ns"].append(nn)
+        o["imm"].append(s.adapt_state.inverse_mass_matrix)
+        o["msq"].append(s.adapt_state.mass_matrix_sqrt)
+    return g, {k: np.array(v) for k, v in o.items()}
+
+
+def test_dense_mass_fixed_matrix_matches_oracle():
+    # dense inverse mass matrix given, no adaptation: the initial
+    # factorisation (hmc_util.py:490-507) and every dense product must agree
+    N, D, C = 200, 4, 2
+    X, y = _problem(N, D, 41)
+    z0 = np.random.default_rng(9).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(23), C)[c] for c in range(C)])
+    imm = _spd(D, 5)
+    kw = dict(dense_mass=True, adapt_step_size=False, adapt_mass_matrix=False,
+              step_size=0.3, inverse_mass_matrix=imm)
+    g, o = _run_pair(X, y, z0, keys, 10, 30, **kw)
+    np.testing.assert_allclose(g["msq"], o["msq"], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g["imm"], o["imm"], rtol=1e-6)
+    assert np.array_equal(g["ns"], o["ns"]), (g["ns"], o["ns"])
+    np.testing.assert_allclose(g["z"], o["z"], rtol=1e-3, atol=1e-3)
+
+
+def test_dense_mass_adaptation_matches_oracle():
+    # dense Welford over the middle window [9, 53] of a 60-step warmup, the
+    # regularised covariance and its factors (hmc_util.py:133-239)
+    N, D, C = 200, 4, 2
+    X, y = _problem(N, D, 43)
+    z0 = np.random.default_rng(10).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(29), C)[c] for c in range(C)])
+    g, o = _run_pair(X, y, z0, keys, 60, 10, dense_mass=True)
+    assert np.array_equal(g["ns"], o["ns"]), (g["ns"], o["ns"])
+    np.testing.assert_allclose(g["z"][:, :25], o["z"][:, :25], rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(g["imm"], o["imm"], rtol=2e-2, atol=1e-4)
+    np.testing.assert_allclose(g["msq"], o["msq"], rtol=2e-2, atol=1e-3)
+    # S S^T = M = (M^-1)^-1 for the engine's own factors
+    for c in range(C):
+        m = np.linalg.inv(g["imm"][c].astype(np.float64))
+        np.testing.assert_allclose(g["msq"][c] @ g["msq"][c].T, m, rtol=1e-3, atol=1e-3)
+        assert np.allclose(np.triu(g["msq"][c], 1), 0)
