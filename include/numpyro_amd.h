@@ -30,7 +30,7 @@ extern "C" {
 
 #define NAX_X_F32 0
 #define NAX_X_BF16 1
-#define NAX_X_FP8 2
+#define NAX_X_FP8 2 /* reserved: refused with NAX_EINVAL */
 
 typedef struct nax_sampler nax_sampler;
 
@@ -112,8 +112,8 @@ void nax_destroy(nax_sampler* h);
  * Bernoulli(logits=X @ coefs)), replacing potential_energy
  * (numpyro/infer/util.py:333-358) + jax.value_and_grad (hmc_util.py:242-252).
  * d_x: [n, d] row-major fp32; d_y: [n] fp32 in {0,1}.  x_dtype selects the
- * storage the kernels read (fp32, bf16 or fp8 e4m3 with per-column scales);
- * the library keeps its own converted copy for bf16/fp8. */
+ * storage the kernels read (fp32, or bf16 on the matrix cores); the library
+ * keeps its own converted copy for bf16. */
 int nax_model_logistic(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                        int32_t d, double prior_scale, int32_t x_dtype);
 
@@ -144,7 +144,7 @@ int nax_model_hier(nax_sampler* h, const float* d_x, const float* d_y, const int
 
 /* nax_model_logistic with the kernel choice for fp32 storage: 0 = auto
  * (matrix cores from 32 chains up), 1 = VALU tile kernel, 2 = fp32 MFMA.
- * bf16 / fp8 storage always use the matrix cores. */
+ * bf16 storage always uses the matrix cores. */
 int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                           int32_t d, double prior_scale, int32_t x_dtype, int32_t kernel);
 

@@ -84,8 +84,8 @@ struct HierPipe {
 // accesses of one wavefront complete in order), and buffer t & 1 is next
 // written (tile t + 2) after every wave has passed the barrier of tile t + 1.
 template <int K>
-__device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, float* gout,
-                                  HierShared<K>& sh) {
+__device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, int ldz,
+                                  float* gout, int ldg, HierShared<K>& sh) {
   using P = HierPipe<K>;
   constexpr int NT = P::NT, EPT = P::EPT, PF = P::PF;
   const int w = threadIdx.x >> 6;
@@ -97,8 +97,8 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   const float* X = hd.x + hd.elt_off[i];
   const float* Y = hd.y + hd.row_off[i];
   const int c = i * K + w;
-  const float* z = zreq + (size_t)c * hd.Dp;
-  float* g = gout + (size_t)c * hd.Dp;
+  const float* z = zreq + (size_t)c * ldz;
+  float* g = gout + (size_t)c * ldg;
   const float mu = z[D];
   const float tau = expf(z[D + 1]);
   const float eta = lane < D ? z[lane] : 0.0f;
@@ -212,11 +212,11 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
 
 template <int K>
 __global__ void __launch_bounds__(64 * K)
-hier_eval_kernel(HierData hd, const float* __restrict__ zreq, double* __restrict__ pe,
-                 float* __restrict__ gout) {
+hier_eval_kernel(HierData hd, const float* __restrict__ zreq, int ldz, double* __restrict__ pe,
+                 float* __restrict__ gout, int ldg) {
   __shared__ HierShared<K> sh;
   const int i = hd.order[blockIdx.x];
-  const double u = hier_group_eval<K>(hd, i, zreq, gout, sh);
+  const double u = hier_group_eval<K>(hd, i, zreq, ldz, gout, ldg, sh);
   if ((threadIdx.x & 63) == 0) pe[i * K + (threadIdx.x >> 6)] = u;
 }
 
@@ -253,7 +253,7 @@ hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
     __threadfence_block();
     __syncthreads();
     if (!need_any) break;
-    const double u = hier_group_eval<K>(hd, i, zreq, gout, sh);
+    const double u = hier_group_eval<K>(hd, i, zreq, cfg.Dp, gout, cfg.Dp, sh);
     if (need) pe = u;
     __threadfence_block();
   }
@@ -291,10 +291,10 @@ struct HierPotential : Potential {
 
   hipError_t eval(hipStream_t s, const float* z, int ldz, int Cn, double* pe, float* g,
                   int ldg) override {
-    if (Cn != C || ldz != hd.Dp || ldg != hd.Dp) return hipErrorInvalidValue;
+    if (Cn != C || ldz < maxd + 2 || ldg < maxd + 2) return hipErrorInvalidValue;
     mark_begin(s);
     switch (hd.K) {
-#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_eval_kernel<KV>, dim3(groups), dim3(64 * KV), 0, s, hd, z, pe, g); break;
+#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_eval_kernel<KV>, dim3(groups), dim3(64 * KV), 0, s, hd, z, ldz, pe, g, ldg); break;
       NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
 #undef NAX_K
       default: return hipErrorInvalidValue;

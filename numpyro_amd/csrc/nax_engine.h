@@ -52,7 +52,16 @@ struct ChainCtx {
   __device__ float* v(int slot) const { return vb + (size_t)slot * Dp; }
 };
 
-#define FOR_D(d) for (int d = cx.lane; d < cx.D; d += NAX_WAVE)
+// A translation unit whose chains have long latent vectors (thousands of
+// entries per lane group) defines NAX_FOR_D_UNROLL before including this
+// header, so that several of a lane's loads are in flight at once; the
+// default keeps register pressure low for short vectors.
+#ifndef NAX_FOR_D_UNROLL
+#define NAX_FOR_D_UNROLL 1
+#endif
+#define NAX_PRAGMA_(x) _Pragma(#x)
+#define NAX_PRAGMA(x) NAX_PRAGMA_(x)
+#define FOR_D(d) NAX_PRAGMA(unroll NAX_FOR_D_UNROLL) for (int d = cx.lane; d < cx.D; d += NAX_WAVE)
 
 // Makes this wavefront's earlier global writes visible to its other lanes.
 __device__ __forceinline__ void wave_sync_mem() { __threadfence_block(); }

@@ -27,6 +27,8 @@
 #include <string>
 
 #include "nax_device.h"
+// T + 2 latent entries per chain: ~47 per lane at T = 3000
+#define NAX_FOR_D_UNROLL 8
 #include "nax_engine.h"
 #include "nax_internal.h"
 
@@ -73,6 +75,7 @@ struct SVModel {
     const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
     const float inv_s2 = (float)(1.0 / (sig * sig));
     double U = 0.0, S2 = 0.0, dnu = 0.0;
+#pragma unroll 4
     for (int t = lane; t < T; t += NAX_WAVE) {
       const float s = z[1 + t];
       const float dprev = s - (t > 0 ? z[t] : 0.0f);
@@ -113,11 +116,13 @@ struct SVModel {
     const int t1 = min(T, t0 + L);
     // pass 1: segment sums -> prefix checkpoints
     double seg = 0.0;
+#pragma unroll 8
     for (int t = t0; t < t1; ++t) seg += (double)zeta[t];
     const double cp = wave_inclusive_scan(seg, lane) - seg;
     // pass 2: StudentT terms and dU/ds_t = 1 - (nu+1) y^2 / (nu + y^2)
     double U = 0.0, dnu = 0.0, A = 0.0, Asc = 0.0, Z2 = 0.0;
     double c = cp;
+#pragma unroll 4
     for (int t = t0; t < t1; ++t) {
       const float zt = zeta[t];
       c += (double)zt;
@@ -138,6 +143,7 @@ struct SVModel {
     // pass 3: dU/dzeta_t = zeta_t + sigma * (suffix of dU/ds from t)
     double P = 0.0;
     c = cp;
+#pragma unroll 4
     for (int t = t0; t < t1; ++t) {
       const float zt = zeta[t];
       c += (double)zt;

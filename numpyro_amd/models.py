@@ -44,8 +44,11 @@ class LogisticRegression(Model):
     (``examples/covtype.py:66-71``, ``test/infer/test_mcmc.py:104-168``).
 
     ``X`` [N, D] and ``y`` [N] are moved to the GPU as float32.  ``x_dtype``
-    picks the storage the gradient kernels read: ``"fp32"`` (plain), ``"bf16"``
-    (MFMA) or ``"fp8"`` (MFMA, e4m3 with per-column scales)."""
+    picks the storage the gradient kernels read: ``"fp32"`` (plain) or
+    ``"bf16"`` (MFMA).  ``"fp8"`` is refused: both operands of an fp8 MFMA are
+    8-bit, so the positions would need a three-term e4m3 split (as costly as
+    the two bf16 products) and the residuals of X^T R would keep 3 mantissa
+    bits."""
 
     def __init__(self, X, y, prior_scale=1.0, x_dtype="fp32", kernel="auto"):
         X = torch.as_tensor(X)
