@@ -144,9 +144,12 @@ def main():
     log(f"{args.x_dtype}: warmup {warm_s:.1f}s, timed {wall:.2f}s, grad evals {ge}, "
         f"{gps:.4g}/s, min ESS {mess:.1f}, max R-hat {rhat:.3f}, mean tree {ns.mean():.1f}, "
         f"accept {acc:.3f}, divergences {div}")
+    # Every tick evaluates only the chains that asked for a gradient (the
+    # compacted request list), so the algorithmic work of the timed launches
+    # is 4 N D per evaluation times the timed gradient evaluations.
     k_avg_ms = kms / max(kn, 1)
-    flops = model.algorithmic_flops(C)
-    achieved = flops / (k_avg_ms * 1e-3) / 1e12
+    flops = model.algorithmic_flops(1) * ge / max(kn, 1)
+    achieved = model.algorithmic_flops(1) * ge / (kms * 1e-3) / 1e12
     peak = PEAKS[args.x_dtype]
     post_state = eng.get_state()
 
@@ -202,7 +205,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic_bytes(args, N, D, C),
                      "kernel": model_kernel_name(args.x_dtype), "kernel_avg_ms": k_avg_ms,
-                     "launches": kn, "algorithmic_flops_per_launch": flops},
+                     "launches": kn, "algorithmic_flops_per_launch": flops,
+                     "traffic_per": "launch over all C chains (profile_kernel); a tick's "
+                                    "launch covers only the chains that asked for a gradient"},
         "fp32_form": fp32,
         "bf16_vs_fp32_min_ess_per_s": (mess / wall) / fp32["min_ess_per_s"]
         if fp32 and fp32["min_ess_per_s"] > 0 else None,

@@ -130,14 +130,29 @@ logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int6
 __global__ void __launch_bounds__(256)
 logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB, int C,
              int D, const float* __restrict__ Z, int ldz, double inv_s2, double log_norm,
-             double u_offset, double* __restrict__ pe, float* __restrict__ g, int ldg) {
-  const int c = blockIdx.x;
+             double u_offset, double* __restrict__ pe, float* __restrict__ g, int ldg,
+             const int* __restrict__ act, const int* __restrict__ nact, int nblk,
+             int64_t ntile) {
+  // workspace column ci = blockIdx.x holds chain act[ci] (ci itself when act
+  // is null); columns past *nact were not computed.  nblk > 0: the MFMA
+  // kernels' split of an nblk-workgroup launch over the requesting chains
+  // (RB row splits, row stride CT 128); nblk = 0: RB as given, stride C.
+  const int ci = blockIdx.x;
+  const int n_act = act ? *nact : C;
+  if (ci >= n_act) return;
+  const int c = act ? act[ci] : ci;
+  int ldw = C;
+  if (nblk > 0) {
+    int CT;
+    logit_split(n_act, nblk, ntile, CT, RB);
+    ldw = CT * 128;
+  }
   const int t = threadIdx.x;
   __shared__ double red[256];
   double pr = 0.0;
   for (int d = t; d < D; d += 256) {
     float s = 0.0f;
-    for (int r = 0; r < RB; ++r) s += wsg[((size_t)r * C + c) * D + d];
+    for (int r = 0; r < RB; ++r) s += wsg[((size_t)r * ldw + ci) * D + d];
     const float zv = Z[(size_t)c * ldz + d];
     g[(size_t)c * ldg + d] = s + (float)(zv * inv_s2);
     pr += 0.5 * (double)zv * zv * inv_s2 + log_norm;
@@ -150,7 +165,7 @@ logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int 
   }
   if (t == 0) {
     double u = 0.0;
-    for (int r = 0; r < RB; ++r) u += wsu[(size_t)r * C + c];
+    for (int r = 0; r < RB; ++r) u += wsu[(size_t)r * ldw + ci];
     pe[c] = u + red[0] + u_offset;
   }
 }
@@ -225,7 +240,7 @@ struct LogisticPotential : Potential {
     const double inv_s2 = 1.0 / (prior_scale * prior_scale);
     const double log_norm = std::log(std::sqrt(2.0 * M_PI) * prior_scale);
     hipLaunchKernelGGL(logit_reduce, dim3(C), dim3(256), 0, s, wsg, wsu, RB, C, d, z, ldz,
-                       inv_s2, log_norm, 0.0, pe, g, ldg);
+                       inv_s2, log_norm, 0.0, pe, g, ldg, nullptr, nullptr, 0, 0);
     return hipGetLastError();
   }
 };

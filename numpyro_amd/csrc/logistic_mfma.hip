@@ -127,8 +127,8 @@ template <typename F, int DP>
 __global__ void __launch_bounds__(256, 1)
 logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __restrict__ B2,
            const float* __restrict__ ypad, int64_t N, int64_t ntile, const float* __restrict__ Z,
-           int ldz, int D, int C, int CT, int RB, float* __restrict__ wsg,
-           double* __restrict__ wsu) {
+           int ldz, int D, int C, float* __restrict__ wsg,
+           double* __restrict__ wsu, const int* __restrict__ act, const int* __restrict__ nact) {
   constexpr int KS = F::KS;
   constexpr int KK = DP / KS;   // GEMM1 K steps
   constexpr int DT = DP / 32;   // GEMM2 d tiles
@@ -149,6 +149,16 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   const int l = tid & 63;
   const int cl = l & 31, h = l >> 5;
   // XCD-aware block placement: blocks of one row split share an XCD.
+  // Chains are taken through the compacted list act[0..*nact) of the chains
+  // that requested a gradient (all C in order when act is null); column
+  // index ci addresses the workspaces (row stride CT * 128), chain act[ci]
+  // the positions.  The chain tiles and row splits are laid out for the
+  // requesting chains only, over the whole launch.
+  const int n_act = act ? *nact : C;
+  if (n_act <= 0) return;
+  int CT, RB;
+  logit_split(n_act, (int)gridDim.x, ntile, CT, RB);
+  const int ldw = CT * 128;
   const int bid = blockIdx.x;
   const int xcd = bid & 7;
   const int slot = bid >> 3;
@@ -156,8 +166,8 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   const int rb = (slot / CT) * 8 + xcd;
   if (rb >= RB) return;  // uniform over the block
   const int c0 = ct * 128 + wave * 32;
-  const int c = c0 + cl;
-  const bool cvalid = c < C;
+  const bool cvalid = c0 + cl < n_act;
+  const int c = cvalid ? (act ? act[c0 + cl] : c0 + cl) : 0;
 
   // Z fragments (B operand of GEMM1): lane (c, h) holds Z[c][KS kk + KS/2 h + j]
   frag zh[KK];
@@ -290,11 +300,11 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int ci = c0 + 8 * (v / 4) + 4 * h + (v % 4);
-      if (ci < C && d < D) wsg[((size_t)rb * C + ci) * D + d] = g[dt][v];
+      if (ci < n_act && d < D) wsg[((size_t)rb * ldw + ci) * D + d] = g[dt][v];
     }
   }
   const double ut = u + __shfl_xor(u, 32, 64);
-  if (h == 0 && cvalid) wsu[(size_t)rb * C + c] = ut;
+  if (h == 0 && cvalid) wsu[(size_t)rb * ldw + c0 + cl] = ut;
 }
 
 // ---- pipelined BF16 kernel --------------------------------------------------------
@@ -312,8 +322,9 @@ template <int DP>
 __global__ void __launch_bounds__(256, 1)
 logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
                 const float* __restrict__ ypad, int64_t N, int64_t ntile,
-                const float* __restrict__ Z, int ldz, int D, int C, int CT, int RB,
-                float* __restrict__ wsg, double* __restrict__ wsu) {
+                const float* __restrict__ Z, int ldz, int D, int C,
+                float* __restrict__ wsg, double* __restrict__ wsu,
+                const int* __restrict__ act, const int* __restrict__ nact) {
   constexpr int KK = DP / 16;
   constexpr int DT = DP / 32;
   constexpr int ABYTES = KK * 64 * 16;
@@ -329,6 +340,16 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   const int wave = tid >> 6;
   const int l = tid & 63;
   const int cl = l & 31, h = l >> 5;
+  // Chains are taken through the compacted list act[0..*nact) of the chains
+  // that requested a gradient (all C in order when act is null); column
+  // index ci addresses the workspaces (row stride CT * 128), chain act[ci]
+  // the positions.  The chain tiles and row splits are laid out for the
+  // requesting chains only, over the whole launch.
+  const int n_act = act ? *nact : C;
+  if (n_act <= 0) return;
+  int CT, RB;
+  logit_split(n_act, (int)gridDim.x, ntile, CT, RB);
+  const int ldw = CT * 128;
   const int bid = blockIdx.x;
   const int xcd = bid & 7;
   const int slot = bid >> 3;
@@ -336,8 +357,8 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   const int rb = (slot / CT) * 8 + xcd;
   if (rb >= RB) return;  // uniform over the block
   const int c0 = ct * 128 + wave * 32;
-  const int c = c0 + cl;
-  const bool cvalid = c < C;
+  const bool cvalid = c0 + cl < n_act;
+  const int c = cvalid ? (act ? act[c0 + cl] : c0 + cl) : 0;
 
   bf16x8 zh[KK], zlo[KK];
 #pragma unroll
@@ -560,11 +581,11 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int ci = c0 + 8 * (v / 4) + 4 * h + (v % 4);
-      if (ci < C && d < D) wsg[((size_t)rb * C + ci) * D + d] = g[dt][v];
+      if (ci < n_act && d < D) wsg[((size_t)rb * ldw + ci) * D + d] = g[dt][v];
     }
   }
   const double ut = u + __shfl_xor(u, 32, 64);
-  if (h == 0 && cvalid) wsu[(size_t)rb * C + c] = ut;
+  if (h == 0 && cvalid) wsu[(size_t)rb * ldw + c0 + cl] = ut;
 }
 
 __global__ void abs_max_kernel(const float* __restrict__ X, int64_t n, unsigned* __restrict__ out) {
@@ -580,7 +601,8 @@ __global__ void abs_max_kernel(const float* __restrict__ X, int64_t n, unsigned*
 __global__ void logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB,
                              int C, int D, const float* __restrict__ Z, int ldz, double inv_s2,
                              double log_norm, double u_offset, double* __restrict__ pe,
-                             float* __restrict__ g, int ldg);
+                             float* __restrict__ g, int ldg, const int* __restrict__ act,
+                             const int* __restrict__ nact, int nblk, int64_t ntile);
 
 struct LogisticMfma : Potential {
   int64_t n = 0, ntile = 0;
@@ -647,11 +669,20 @@ struct LogisticMfma : Potential {
 
   hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe, float* g,
                   int ldg) override {
+    return eval_active(s, z, ldz, C, nullptr, nullptr, pe, g, ldg);
+  }
+  // Only the chains act[0..*nact) (device memory) are evaluated; launch
+  // geometry stays that of all C chains, tiles past *nact exit at once.
+  hipError_t eval_active(hipStream_t s, const float* z, int ldz, int C, const int* act,
+                         const int* nact, double* pe, float* g, int ldg) override {
     const int CT = (C + 127) / 128;
     // one 4-wave block per CU; row splits rounded to a multiple of the 8 XCDs
     int RB = std::max(8, (num_cu / CT) / 8 * 8);
     RB = (int)std::min<int64_t>(RB, (ntile + 7) / 8 * 8);
-    const size_t need = (size_t)RB * C * d;
+    const int nblk = CT * RB;
+    // the kernels re-split the launch for the requesting chains: RB' CT' <=
+    // nblk, row stride CT' 128, so nblk * 128 columns always suffice
+    const size_t need = (size_t)nblk * 128 * d;
     if (need > ws_elems) {
       hipFree(wsg);
       hipFree(wsu);
@@ -659,16 +690,15 @@ struct LogisticMfma : Potential {
       wsu = nullptr;
       hipError_t e = hipMalloc(&wsg, sizeof(float) * need);
       if (e != hipSuccess) return e;
-      e = hipMalloc(&wsu, sizeof(double) * (size_t)RB * C);
+      e = hipMalloc(&wsu, sizeof(double) * (size_t)nblk * 128);
       if (e != hipSuccess) return e;
       ws_elems = need;
     }
-    const int nblk = CT * RB;
     mark_begin(s);
 #define NAX_L(FORM, DPV)                                                                      \
   hipLaunchKernelGGL((logit_mfma<FORM, DPV>), dim3(nblk), dim3(256), 0, s,                    \
                      (const FORM::frag*)a1, (const FORM::frag*)b2, ypad, n, ntile, z, ldz, d, C, \
-                     CT, RB, wsg, wsu)
+                     wsg, wsu, act, nact)
 #define NAX_DP(FORM)                    \
   switch (dp) {                         \
     case 32: NAX_L(FORM, 32); break;    \
@@ -683,7 +713,7 @@ struct LogisticMfma : Potential {
   }
 #define NAX_P(DPV)                                                                             \
   hipLaunchKernelGGL((logit_bf16_pipe<DPV>), dim3(nblk), dim3(256), 0, s, (const bf16x8*)a1,   \
-                     (const f16x8*)b2, ypad, n, ntile, z, ldz, d, C, CT, RB, wsg, wsu)
+                     (const f16x8*)b2, ypad, n, ntile, z, ldz, d, C, wsg, wsu, act, nact)
     if (bf16) {
       switch (dp) {
         case 32: NAX_P(32); break;
@@ -708,7 +738,7 @@ struct LogisticMfma : Potential {
     // padded rows each added fp32(ln 2) to every chain's energy
     const double pad_u = (double)(ntile * 32 - n) * (double)0.693147180559945309f;
     hipLaunchKernelGGL(logit_reduce, dim3(C), dim3(256), 0, s, wsg, wsu, RB, C, d, z, ldz,
-                       inv_s2, log_norm, -pad_u, pe, g, ldg);
+                       inv_s2, log_norm, -pad_u, pe, g, ldg, act, nact, nblk, ntile);
     return hipGetLastError();
   }
 };

@@ -149,6 +149,19 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Work split of the MFMA logistic kernels for n_act requesting chains over a
+// launch of nblk workgroups: CT tiles of 128 chains times RB row splits (a
+// multiple of the 8 XCDs, at most one split per 8 row tiles), so that the
+// whole launch stays busy however few chains ask for a gradient.
+__host__ __device__ __forceinline__ void logit_split(int n_act, int nblk, int64_t ntile, int& CT,
+                                                     int& RB) {
+  CT = n_act > 0 ? (n_act + 127) / 128 : 1;
+  int rb = (nblk / CT) / 8 * 8;
+  rb = rb < 8 ? 8 : rb;
+  const int64_t cap = (ntile + 7) / 8 * 8;
+  RB = rb < cap ? rb : (int)cap;
+}
+
 __device__ __forceinline__ double log_add_exp(double a, double b) {
   double m = fmax(a, b);
   if (m == -INFINITY) return -INFINITY;

@@ -749,14 +749,17 @@ perchain_eval_kernel(M m, const float* __restrict__ z, int ldz, int C, double* _
 
 // The whole sampling loop of one chain in one launch: the wave alternates
 // advance_chain() and its own potential until the chain reaches `target`.
-template <class M, bool DENSE>
-__global__ void __launch_bounds__(256)
+// WPB wavefronts (chains) per workgroup: one, for latency-bound chains with
+// long vectors, spreads the chains over as many CUs (and their L1 caches and
+// address units) as there are chains.
+template <class M, bool DENSE, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
                   ChainState* __restrict__ states, float* __restrict__ vec,
                   double* __restrict__ welf, float* __restrict__ zreq, float* __restrict__ gout,
                   double* __restrict__ peout, int64_t* __restrict__ grad_evals) {
-  __shared__ ChainState sS[4];
-  const int c = wave_chain(4);
+  __shared__ ChainState sS[WPB];
+  const int c = wave_chain(WPB);
   if (c >= cfg.C) return;
   const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
   float* zr = zreq + (size_t)c * cfg.Dp;

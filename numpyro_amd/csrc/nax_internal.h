@@ -86,6 +86,12 @@ struct Potential {
   // z: [C, ldz] fp32; writes pe[C] (fp64) and g: [C, ldg] fp32.
   virtual hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe,
                           float* g, int ldg) = 0;
+  // The same restricted to the chains act[0..*nact) (device memory, written
+  // by the tick kernel); a potential may evaluate more chains than asked.
+  virtual hipError_t eval_active(hipStream_t s, const float* z, int ldz, int C, const int* act,
+                                 const int* nact, double* pe, float* g, int ldg) {
+    return eval(s, z, ldz, C, pe, g, ldg);
+  }
   // Bytes / flops one eval of C chains moves / does (algorithmic figures).
   virtual double alg_bytes(int C) const { return 0; }
   virtual double alg_flops(int C) const { return 0; }

@@ -35,7 +35,8 @@ __global__ void __launch_bounds__(256)
 advance_kernel(EngineCfg cfg, Collect col, int64_t target, ChainState* __restrict__ states,
                float* __restrict__ vec, double* __restrict__ welf, float* __restrict__ zreq,
                const float* __restrict__ gout, const double* __restrict__ peout,
-               int* __restrict__ pending, int64_t* __restrict__ grad_evals) {
+               int* __restrict__ pending, int* __restrict__ active,
+               int64_t* __restrict__ grad_evals) {
   __shared__ ChainState sS[4];
   const int c = wave_chain(4);
   if (c >= cfg.C) return;
@@ -47,7 +48,7 @@ advance_kernel(EngineCfg cfg, Collect col, int64_t target, ChainState* __restric
                                        gout + (size_t)c * cfg.Dp, peout[c], evals);
   copy_state(states[c], S, cx.lane);
   if (cx.lane == 0) {
-    if (need_grad) atomicAdd(pending, 1);
+    if (need_grad) active[atomicAdd(pending, 1)] = c;  // compacted request list
     if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
   }
 }
@@ -224,6 +225,7 @@ struct nax_sampler {
   float* gout = nullptr;
   double* peout = nullptr;
   int* ctr = nullptr;          // device counters [kBatch]
+  int* active = nullptr;       // device [C]: chains that requested a gradient this tick
   int64_t* gevals = nullptr;   // device
   int* ctr_host = nullptr;     // pinned
   int64_t* gevals_host = nullptr;
@@ -342,6 +344,7 @@ extern "C" int nax_create(const nax_config* c, nax_sampler** out) {
   NAX_HIP(hipMalloc(&h->gout, C * Dp * sizeof(float)));
   NAX_HIP(hipMalloc(&h->peout, C * sizeof(double)));
   NAX_HIP(hipMalloc(&h->ctr, kBatch * sizeof(int)));
+  NAX_HIP(hipMalloc(&h->active, C * sizeof(int)));
   NAX_HIP(hipMalloc(&h->gevals, sizeof(int64_t)));
   NAX_HIP(hipHostMalloc(&h->ctr_host, sizeof(int) * 2, hipHostMallocDefault));
   NAX_HIP(hipHostMalloc(&h->gevals_host, sizeof(int64_t), hipHostMallocDefault));
@@ -358,7 +361,7 @@ extern "C" void nax_destroy(nax_sampler* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   delete h->pot;
   hipFree(h->vec); hipFree(h->wel); hipFree(h->st); hipFree(h->zreq); hipFree(h->gout);
-  hipFree(h->peout); hipFree(h->ctr); hipFree(h->gevals);
+  hipFree(h->peout); hipFree(h->ctr); hipFree(h->gevals); hipFree(h->active);
   hipFree(h->cfg.dimm_t); hipFree(h->cfg.dmsq_t); hipFree(h->cfg.dm2);
   hipHostFree(h->ctr_host); hipHostFree(h->gevals_host);
   for (auto e : h->ev) hipEventDestroy(e);
@@ -455,11 +458,16 @@ static int eval_events(nax_sampler* h, int k) {
   return NAX_OK;
 }
 
-static int eval_potential(nax_sampler* h, int k = 0) {
+// compact: evaluate only the chains the tick kernel of batch slot k listed.
+static int eval_potential(nax_sampler* h, int k = 0, bool compact = false) {
   const EngineCfg& e = h->cfg;
   int rc = eval_events(h, k);
   if (rc) return rc;
-  NAX_HIP(h->pot->eval(h->stream, h->zreq, e.Dp, e.C, h->peout, h->gout, e.Dp));
+  if (compact)
+    NAX_HIP(h->pot->eval_active(h->stream, h->zreq, e.Dp, e.C, h->active, h->ctr + k, h->peout,
+                                h->gout, e.Dp));
+  else
+    NAX_HIP(h->pot->eval(h->stream, h->zreq, e.Dp, e.C, h->peout, h->gout, e.Dp));
   h->pot->ev_begin = h->pot->ev_end = nullptr;
   return NAX_OK;
 }
@@ -498,11 +506,11 @@ static int launch_advance(nax_sampler* h, const Collect& col, int64_t target, in
   dim3 grid((e.C + 3) / 4);
   if (e.dense)
     hipLaunchKernelGGL(advance_kernel<true>, grid, dim3(256), 0, h->stream, e, col, target,
-                       h->st, h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot,
+                       h->st, h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot, h->active,
                        h->gevals);
   else
     hipLaunchKernelGGL(advance_kernel<false>, grid, dim3(256), 0, h->stream, e, col, target,
-                       h->st, h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot,
+                       h->st, h->vec, h->wel, h->zreq, h->gout, h->peout, h->ctr + slot, h->active,
                        h->gevals);
   NAX_HIP(hipGetLastError());
   return NAX_OK;
@@ -539,7 +547,7 @@ extern "C" int nax_init(nax_sampler* h, const float* z0, const uint32_t* keys,
       NAX_HIP(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(int), hipMemcpyDeviceToHost, h->stream));
       NAX_HIP(hipStreamSynchronize(h->stream));
       if (h->ctr_host[0] == 0) break;
-      rc = eval_potential(h);
+      rc = eval_potential(h, 0, true);
       if (rc) return rc;
       hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, h->stream, h->ctr, kBatch);
       rc = launch_advance(h, col, 0, 0);
@@ -591,7 +599,7 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
     for (int k = 0; k < kBatch; ++k) {
       int rc = launch_advance(h, col, target, k);
       if (rc) return rc;
-      rc = eval_potential(h, k);
+      rc = eval_potential(h, k, true);
       if (rc) return rc;
     }
     NAX_HIP(hipMemcpyAsync(h->ctr_host, h->ctr + kBatch - 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));

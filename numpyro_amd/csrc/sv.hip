@@ -191,12 +191,13 @@ struct SVPotential : Potential {
                             int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
                             float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
+    // one chain per workgroup: each chain gets a CU to itself while C <= #CUs
     if (cfg.dense)
-      hipLaunchKernelGGL((persistent_kernel<SVModel, true>), dim3((cfg.C + 3) / 4), dim3(256), 0,
-                         s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+      hipLaunchKernelGGL((persistent_kernel<SVModel, true, 1>), dim3(cfg.C), dim3(64), 0, s, m,
+                         cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
     else
-      hipLaunchKernelGGL((persistent_kernel<SVModel, false>), dim3((cfg.C + 3) / 4), dim3(256),
-                         0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+      hipLaunchKernelGGL((persistent_kernel<SVModel, false, 1>), dim3(cfg.C), dim3(64), 0, s, m,
+                         cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
     mark_end(s);
     return hipGetLastError();
   }

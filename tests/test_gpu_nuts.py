@@ -30,11 +30,11 @@ def _problem(N, D, seed):
     return X, y
 
 
-def _run_engine(X, y, z0, keys, num_warmup, num_samples, algo="NUTS", **kw):
+def _run_engine(X, y, z0, keys, num_warmup, num_samples, algo="NUTS", model_kw=None, **kw):
     from numpyro_amd.infer import hmc
     from numpyro_amd.infer.util import fori_collect
     from numpyro_amd.models import LogisticRegression
-    m = LogisticRegression(X, y)
+    m = LogisticRegression(X, y, **(model_kw or {}))
     init_kernel, sample_kernel = hmc(m, algo=algo)
     st = init_kernel(torch.from_numpy(z0), num_warmup, rng_key=keys, **kw)
     out = fori_collect(0, num_warmup + num_samples, sample_kernel, st,
@@ -231,3 +231,19 @@ def test_dense_mass_adaptation_matches_oracle():
         m = np.linalg.inv(g["imm"][c].astype(np.float64))
         np.testing.assert_allclose(g["msq"][c] @ g["msq"][c].T, m, rtol=1e-3, atol=1e-3)
         assert np.allclose(np.triu(g["msq"][c], 1), 0)
+
+
+def test_compacted_mfma_chains_match_oracle():
+    # 160 chains on the fp32 MFMA kernel: two chain tiles, and from the first
+    # chain to finish an iteration batch on, each tick evaluates only the
+    # compacted list of chains that asked for a gradient
+    N, D, C = 200, 5, 160
+    X, y = _problem(N, D, 51)
+    z0 = np.random.default_rng(12).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(31), C)[c] for c in range(C)])
+    kw = dict(adapt_step_size=False, adapt_mass_matrix=False, step_size=0.2)
+    gz, gns, gpe, _ = _run_engine(X, y, z0, keys, 0, 12, model_kw=dict(kernel="mfma"), **kw)
+    oz, ons, ope, _ = _run_oracle(X, y, z0, keys, 0, 12, **kw)
+    assert np.array_equal(gns, ons), np.argwhere(gns != ons)[:5]
+    np.testing.assert_allclose(gz, oz, rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(gpe, ope, rtol=1e-5)
