@@ -179,7 +179,9 @@ def main():
             log(f"cpu baseline skipped: {e!r}")
 
     line = {
-        "metric": "min-ESS/sec + grad evals/sec, 1024-chain logistic NUTS, MFMA vs plain FP32",
+        "metric": ("min-ESS/sec + grad evals/sec, 1024-chain logistic NUTS, MFMA vs plain FP32"
+                   if args.config == "c5" else
+                   f"min-ESS/sec + grad evals/sec, {C}-chain logistic NUTS ({args.config})"),
         "value": gps,
         "unit": "grad evals/s",
         "n_gpus": 1,
@@ -204,7 +206,7 @@ def main():
         "warmup_s": warm_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic_bytes(args, N, D, C),
-                     "kernel": model_kernel_name(args.x_dtype), "kernel_avg_ms": k_avg_ms,
+                     "kernel": model_kernel_name(args.x_dtype, D, C), "kernel_avg_ms": k_avg_ms,
                      "launches": kn, "algorithmic_flops_per_launch": flops,
                      "traffic_per": "launch over all C chains (profile_kernel); a tick's "
                                     "launch covers only the chains that asked for a gradient"},
@@ -287,8 +289,15 @@ def bench_hier(args):
     print(json.dumps(line), flush=True)
 
 
-def model_kernel_name(x_dtype):
-    return {"bf16": "logit_bf16_pipe<256>", "fp32": "logit_mfma<FormF32,256>"}[x_dtype]
+def model_kernel_name(x_dtype, D, C):
+    dp = (D + 31) // 32 * 32
+    if x_dtype == "bf16":
+        return f"logit_bf16_pipe<{dp}>"
+    if C < 32:
+        return "logit_partial_f32 (VALU tile)"
+    if C > 128:
+        return f"logit_mfma<FormF32,{dp}>"
+    return f"logit_partial_f32 or logit_mfma<FormF32,{dp}> (autotuned)"
 
 
 # HBM bytes per launch of the main kernel measured with rocprofv3 --pmc

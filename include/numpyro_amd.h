@@ -225,6 +225,12 @@ int nax_set_progress(nax_sampler* h, void (*cb)(void* user, int64_t ticks, int32
  * last reset. */
 int nax_kernel_timing(nax_sampler* h, int32_t enable, double* total_ms, int64_t* launches);
 
+/* Launch mode of nax_run's tick loop (no counterpart in the reference, whose
+ * loop is one XLA program): with enable != 0 (the default) batches of ticks
+ * are captured once per call as a hipGraph and replayed; 0 enqueues every
+ * kernel directly.  Results are identical either way. */
+int nax_set_graphs(nax_sampler* h, int32_t enable);
+
 #ifdef __cplusplus
 }
 #endif

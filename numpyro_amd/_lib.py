@@ -83,6 +83,7 @@ SIGNATURES = {
     "nax_random_normal": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nax_chain_keys": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nax_set_progress": (C.c_int, [C.c_void_p, PROGRESS_CB, C.c_void_p, C.c_int64]),
+    "nax_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "nax_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]),
 }
@@ -125,3 +126,10 @@ def ptr(t):
     if t is None:
         return None
     return C.c_void_p(t.data_ptr())
+
+
+def stride(handle):
+    """Leading dimension (Dp) of the engine's per-chain vectors."""
+    ld = C.c_int32()
+    check(load().nax_buffers(handle, None, None, None, C.byref(ld)), "nax_buffers")
+    return ld.value

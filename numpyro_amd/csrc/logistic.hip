@@ -181,6 +181,7 @@ struct LogisticPotential : Potential {
   double* wsu = nullptr;
   size_t ws_chains = 0;
   int rb = 0;
+  int rb_pick = 0;  // row splits chosen by the autotuner (0: default rule)
 
   ~LogisticPotential() override {
     hipFree(wsg);
@@ -212,6 +213,7 @@ struct LogisticPotential : Potential {
     const int ctiles = (C + BC - 1) / BC;
     const int64_t ntiles = (n + BN - 1) / BN;
     int RB = (int)std::min<int64_t>(ntiles, std::max<int64_t>(1, 2048 / ctiles));
+    if (rb_pick > 0) RB = (int)std::min<int64_t>(ntiles, rb_pick);
     hipError_t e = ensure_ws(C, RB);
     if (e != hipSuccess) return e;
     const int dch = (d + 63) / 64;
@@ -259,7 +261,8 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
     return nullptr;
   }
   const bool mfma = fuse_variant == 2 || (fuse_variant == 0 && num_chains >= 32 && d <= 256);
-  if (mfma) return make_logistic_mfma(x, y, n, d, prior_scale, false, s, err);
+  if (mfma && !(fuse_variant == 0 && num_chains <= 128 && d <= 320))
+    return make_logistic_mfma(x, y, n, d, prior_scale, false, s, err);
   if (d > 320) {
     *err = "the fp32 logistic tile kernel supports dim <= 320";
     return nullptr;
@@ -271,7 +274,58 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
   p->d = d;
   p->prior_scale = prior_scale;
   p->x_dtype = x_dtype;
-  return p;
+  if (fuse_variant != 0 || num_chains < 1) return p;
+  // Autotuner ("auto" kernel): row splits of the tile kernel, and for 32 <=
+  // C <= 128 the tile kernel against the fp32 MFMA kernel, each timed over a
+  // few evaluations of all C chains at z = 0; the fastest is kept.
+  const int C = num_chains;
+  const int ld = (d + 63) / 64 * 64;
+  float *z = nullptr, *g = nullptr;
+  double* pe = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = hipMalloc(&z, sizeof(float) * (size_t)C * ld) == hipSuccess &&
+            hipMalloc(&g, sizeof(float) * (size_t)C * ld) == hipSuccess &&
+            hipMalloc(&pe, sizeof(double) * C) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+            hipMemsetAsync(z, 0, sizeof(float) * (size_t)C * ld, s) == hipSuccess;
+  auto time_it = [&](Potential* q) -> float {
+    if (q->eval(s, z, ld, C, pe, g, ld) != hipSuccess) return 1e30f;  // warm-up, workspaces
+    hipEventRecord(e0, s);
+    for (int r = 0; r < 3; ++r) q->eval(s, z, ld, C, pe, g, ld);
+    hipEventRecord(e1, s);
+    float ms = 1e30f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return 1e30f;
+    return ms;
+  };
+  Potential* best = p;
+  if (ok) {
+    const int64_t ntiles = (n + 31) / 32;
+    const int ctiles = (C + 7) / 8;
+    float best_ms = 1e30f;
+    int best_rb = 0;
+    for (int cand : {2048, 1024, 512, 256, 128, 64}) {
+      const int rbv = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles, cand / ctiles));
+      p->rb_pick = rbv;
+      const float ms = time_it(p);
+      if (ms < best_ms) { best_ms = ms; best_rb = rbv; }
+    }
+    p->rb_pick = best_rb;
+    if (mfma) {
+      std::string e2;
+      Potential* q = make_logistic_mfma(x, y, n, d, prior_scale, false, s, &e2);
+      if (q) {
+        if (time_it(q) < best_ms) best = q;
+        else delete q;
+      }
+    }
+  }
+  hipFree(z); hipFree(g); hipFree(pe);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(s);
+  if (best != p) delete p;
+  return best;
 }
 
 }  // namespace nax

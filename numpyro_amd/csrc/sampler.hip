@@ -226,6 +226,7 @@ struct nax_sampler {
   double* peout = nullptr;
   int* ctr = nullptr;          // device counters [kBatch]
   int* active = nullptr;       // device [C]: chains that requested a gradient this tick
+  bool graphs = true;          // replay tick batches as hipGraphs (nax_set_graphs)
   int64_t* gevals = nullptr;   // device
   int* ctr_host = nullptr;     // pinned
   int64_t* gevals_host = nullptr;
@@ -593,13 +594,47 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
     }
     return NAX_OK;
   }
-  int64_t ticks = 0;
-  for (;;) {
+  // One batch = kBatch ticks (tick kernel + potential), enqueued directly or
+  // replayed as a hipGraph captured after two direct batches (so workspaces
+  // are allocated and short calls never pay for a capture).  The graph holds
+  // kernels only (counters are cleared by a kernel, not a memset).  Kernel
+  // timing records events around the main kernel, so it runs uncaptured.
+  auto batch = [&]() -> int {
     hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, h->stream, h->ctr, kBatch);
     for (int k = 0; k < kBatch; ++k) {
       int rc = launch_advance(h, col, target, k);
       if (rc) return rc;
       rc = eval_potential(h, k, true);
+      if (rc) return rc;
+    }
+    return NAX_OK;
+  };
+  const bool use_graph = h->graphs && !h->timing;
+  hipGraphExec_t exec = nullptr;
+  struct ExecGuard {
+    hipGraphExec_t* e;
+    ~ExecGuard() { if (*e) hipGraphExecDestroy(*e); }
+  } guard{&exec};
+  int64_t ticks = 0;
+  for (;;) {
+    if (use_graph && !exec && ticks >= 2 * kBatch) {
+      NAX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = batch();
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+      if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+      }
+      NAX_HIP(ce);
+      const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      NAX_HIP(ie);
+    }
+    if (exec) {
+      NAX_HIP(hipGraphLaunch(exec, h->stream));
+    } else {
+      const int rc = batch();
       if (rc) return rc;
     }
     NAX_HIP(hipMemcpyAsync(h->ctr_host, h->ctr + kBatch - 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -738,6 +773,12 @@ extern "C" int nax_leapfrog(nax_sampler* h, float* z, float* r, float* g, double
     NAX_HIP(hipGetLastError());
   }
   NAX_HIP(hipStreamSynchronize(h->stream));
+  return NAX_OK;
+}
+
+extern "C" int nax_set_graphs(nax_sampler* h, int32_t enable) {
+  if (!h) { set_error("null handle"); return NAX_EINVAL; }
+  h->graphs = enable != 0;
   return NAX_OK;
 }
 

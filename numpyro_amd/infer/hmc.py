@@ -180,6 +180,11 @@ class Engine:
             out["diverging"] = out["diverging"].bool()
         return out
 
+    def set_graphs(self, enable):
+        """Replay nax_run's tick batches as hipGraphs (default) or launch
+        every kernel directly; the draws are identical."""
+        _lib.check(self.lib.nax_set_graphs(self.h, int(bool(enable))), "nax_set_graphs")
+
     def set_progress(self, fn, every_ticks=2048):
         """fn(ticks, pending) is called from the run loop every `every_ticks`
         gradient evaluations; None switches it off."""

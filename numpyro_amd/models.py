@@ -211,3 +211,52 @@ class PotentialFn(Model):
 
 def normal_log_norm(scale):
     return math.log(math.sqrt(2 * math.pi) * scale)
+
+
+# ---- the adjoint kernels behind torch.autograd ---------------------------------
+
+def _device_value_and_grad(model, z):
+    """U [C] (float64) and dU/dz [C, dim] (float32) of a built-in model at
+    z [C, dim] from the library's fused potential + adjoint kernels."""
+    from .infer.hmc import Engine  # local: infer imports this module
+    C = int(z.shape[0])
+    cache = model.__dict__.setdefault("_vg_engines", {})
+    eng = cache.get(C)
+    if eng is None:
+        eng = cache[C] = Engine(model, C, "NUTS", 0)
+    ld = _lib.stride(eng.h)
+    zp = torch.zeros((C, ld), dtype=torch.float32, device="cuda")
+    zp[:, :model.dim] = z.detach().to(device="cuda", dtype=torch.float32)
+    pe = torch.empty(C, dtype=torch.float64, device="cuda")
+    g = torch.zeros((C, ld), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().nax_potential_grad(eng.h, _lib.ptr(zp), ld, C, _lib.ptr(pe),
+                                              _lib.ptr(g)), "nax_potential_grad")
+    return pe, g[:, :model.dim].contiguous()
+
+
+class _PotentialEnergy(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, z, model):
+        pe, g = _device_value_and_grad(model, z)
+        ctx.save_for_backward(g)
+        return pe
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return (grad_out.to(g.dtype)[:, None] * g).to(torch.float32), None
+
+
+def potential_energy(model, z):
+    """``potential_energy`` (``numpyro/infer/util.py:333-358``) of a built-in
+    model for a batch of unconstrained latents z [C, dim], as a differentiable
+    torch function: the forward pass and its gradient both come from the
+    model's hand-written HIP potential + adjoint kernel (one launch computes
+    both; backward only scales the saved gradient).  Returns U [C] float64."""
+    if isinstance(model, PotentialFn):
+        raise ValueError("potential_energy needs a built-in model (HIP kernels)")
+    z = torch.as_tensor(z)
+    if z.ndim != 2 or z.shape[1] != model.dim:
+        raise ValueError(f"z must be [C, {model.dim}]")
+    return _PotentialEnergy.apply(z, model)

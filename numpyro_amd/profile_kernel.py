@@ -20,7 +20,7 @@ import torch
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c4"])
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c4", "c3", "c3nc"])
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "mfma"])
     ap.add_argument("--reps", type=int, default=10)
@@ -35,7 +35,17 @@ def main(argv=None):
 
     out = {}
     if args.reps > 0:
-        if args.config == "c4":
+        if args.config in ("c3", "c3nc"):
+            from .models import StochasticVolatility
+            r, meta = workloads.make_sv(3000, 3)
+            model = StochasticVolatility(r, noncentred=args.config == "c3nc")
+            C_, D = args.chains or 128, model.dim
+            cfg = {"N": 3000, "chains": C_, "D": D}
+            gen = torch.Generator(device="cuda")
+            gen.manual_seed(0)
+            z = 0.1 * torch.randn((C_, D), generator=gen, device="cuda")
+            args.x_dtype = "fp32"
+        elif args.config == "c4":
             from .models import HierarchicalLogistic
             G = args.chains or 1024
             model = HierarchicalLogistic(*workloads.make_hier(G, 4), chains_per_group=4)

@@ -116,3 +116,30 @@ def test_functional_api_normal():
     assert s.shape == (2, 3000, 1)
     np.testing.assert_allclose(s.mean(axis=(1, 2)), [true_mean] * 2, rtol=0.15)
     np.testing.assert_allclose(s.std(axis=(1, 2)), [true_std] * 2, rtol=0.1)
+
+
+def test_graph_replay_matches_direct_launches():
+    # nax_run replays its tick batches as a captured hipGraph (after two
+    # direct batches); with direct launches only, every draw must be the same
+    # bit for bit.  ~60 iterations x ~7 leaves = hundreds of ticks: many
+    # replays, each followed by the loop's own host synchronisation.
+    import numpy as np
+    import torch
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import LogisticRegression
+    rng = np.random.default_rng(5)
+    N, D, C = 500, 8, 64
+    X = rng.normal(size=(N, D)).astype(np.float32)
+    y = (rng.uniform(size=N) < 0.5).astype(np.float32)
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(3), C)
+    outs = []
+    for graphs in (True, False):
+        eng = Engine(LogisticRegression(X, y, kernel="mfma"), C, "NUTS", 30)
+        eng.set_graphs(graphs)
+        eng.init(torch.zeros((C, D)), keys)
+        out = eng.run(60, collect_fields=("z", "num_steps", "potential_energy"), size=60)
+        outs.append({k: v.cpu().numpy() for k, v in out.items()})
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    assert outs[0]["num_steps"].sum() > 16 * 3 * C  # enough ticks for several replays
