@@ -67,11 +67,15 @@ def progress_ticks(model, C):
 
 
 def min_ess(z):
-    from numpyro_amd.diagnostics import effective_sample_size, split_gelman_rubin
-    zz = z.cpu().numpy().astype(np.float64)
-    ess = effective_sample_size(zz)
-    rhat = split_gelman_rubin(zz) if zz.shape[1] >= 4 else np.full(zz.shape[2], np.nan)
-    return float(np.min(ess)), float(np.nanmax(rhat))
+    """min ESS and max split R-hat over coordinates of draws z [C, S, D],
+    computed on the GPU (nax_diagnostics; the reference's formulas)."""
+    from numpyro_amd.diagnostics import device_diagnostics, effective_sample_size
+    if z.shape[1] > 4096:
+        return float(np.min(effective_sample_size(z.cpu().numpy().astype(np.float64)))), \
+            float("nan")
+    ess, rhat = device_diagnostics(z)
+    rhat = rhat.cpu().numpy() if z.shape[1] >= 4 else np.full(z.shape[2], np.nan)
+    return float(ess.min()), float(np.nanmax(rhat))
 
 
 def cpu_baseline(X, y, z0, keys, budget_s, threads):

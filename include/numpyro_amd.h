@@ -143,8 +143,11 @@ int nax_model_hier(nax_sampler* h, const float* d_x, const float* d_y, const int
                    int32_t chains_per_group);
 
 /* nax_model_logistic with the kernel choice for fp32 storage: 0 = auto
- * (matrix cores from 32 chains up), 1 = VALU tile kernel, 2 = fp32 MFMA.
- * bf16 storage always uses the matrix cores. */
+ * (one wavefront per chain in a persistent launch for <= 16 chains on
+ * n*d <= 2^20, d <= 64; otherwise the VALU tile kernel or, from 32 chains
+ * up, the fp32 MFMA kernel, whichever the autotuner times faster), 1 = VALU
+ * tile kernel, 2 = fp32 MFMA, 3 = one wavefront per chain.  bf16 storage
+ * always uses the matrix cores. */
 int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                           int32_t d, double prior_scale, int32_t x_dtype, int32_t kernel);
 
@@ -230,6 +233,14 @@ int nax_kernel_timing(nax_sampler* h, int32_t enable, double* total_ms, int64_t*
  * are captured once per call as a hipGraph and replayed; 0 enqueues every
  * kernel directly.  Results are identical either way. */
 int nax_set_graphs(nax_sampler* h, int32_t enable);
+
+/* Diagnostics of collected draws d_x [chains, draws, params] fp32 (device):
+ * effective_sample_size (numpyro/diagnostics.py:158-203, biased
+ * autocovariance, Geyer's initial monotone sequence) into d_ess [params] and
+ * split_gelman_rubin (diagnostics.py:64-80) into d_split_rhat [params] (may
+ * be NULL), both fp64 on the device.  draws <= 4096.  Synchronous. */
+int nax_diagnostics(const float* d_x, int32_t chains, int32_t draws, int32_t params,
+                    double* d_ess, double* d_split_rhat);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,8 @@ SIGNATURES = {
     "nax_chain_keys": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nax_set_progress": (C.c_int, [C.c_void_p, PROGRESS_CB, C.c_void_p, C.c_int64]),
     "nax_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nax_diagnostics": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                  C.c_void_p]),
     "nax_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]),
 }

@@ -6,7 +6,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnumpyro_amd.so")
-SOURCES = ["sampler.hip", "logistic.hip", "logistic_mfma.hip", "sv.hip", "hier.hip"]
+SOURCES = ["sampler.hip", "logistic.hip", "logistic_mfma.hip", "logistic_chain.hip", "sv.hip",
+           "hier.hip", "diagnostics.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -260,6 +260,11 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
     *err = "x_dtype not supported";
     return nullptr;
   }
+  // a handful of chains on a small design: one wavefront per chain, the
+  // whole sampling loop in one persistent launch
+  if (fuse_variant == 3 ||
+      (fuse_variant == 0 && num_chains <= 16 && d <= 64 && (double)n * d <= (double)(1 << 20)))
+    return make_logistic_chain(x, y, n, d, prior_scale, err);
   const bool mfma = fuse_variant == 2 || (fuse_variant == 0 && num_chains >= 32 && d <= 256);
   if (mfma && !(fuse_variant == 0 && num_chains <= 128 && d <= 320))
     return make_logistic_mfma(x, y, n, d, prior_scale, false, s, err);

@@ -1,0 +1,126 @@
+// Bayesian logistic regression with one wavefront per chain, for few chains
+// on small designs (the reference's CPU-sized configuration): the whole NUTS
+// loop of a chain runs in one persistent launch, the chain's own wave
+// evaluating U and dU (examples/covtype.py:66-71: coefs ~ Normal(0, s),
+// obs ~ Bernoulli(logits = X coefs); BernoulliLogits discrete.py:244-263,
+// binary_cross_entropy_with_logits distributions/util.py:317-320).  With a
+// handful of chains the batched tile kernel cannot fill the GPU and every
+// tick pays a launch round trip; here a leaf costs one wave's pass over X
+// (L2-resident) and no launch.
+//
+// Lane = row: each lane takes rows lane, lane + 64, ... and keeps the
+// chain's position (d <= DMAX) and its own partial X^T r in registers; one
+// wave reduction per coordinate finishes the gradient.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "nax_device.h"
+#include "nax_engine.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+template <int DMAX>
+struct LogisticChainModel {
+  const float* x;  // [n, d] row-major fp32
+  const float* y;  // [n]
+  int64_t n;
+  int d;
+  float inv_s2;
+  double log_norm;  // d * log(sqrt(2 pi) s)
+
+  __device__ double value_and_grad(const float* z, float* g, int lane) const {
+    float zr[DMAX], gp[DMAX];
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) {
+      zr[k] = k < d ? z[k] : 0.0f;
+      gp[k] = 0.0f;
+    }
+    double u = 0.0;
+    for (int64_t row = lane; row < n; row += NAX_WAVE) {
+      const float* xr = x + row * d;
+      float xv[DMAX];
+      float l0 = 0.0f, l1 = 0.0f;
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) {
+        xv[k] = k < d ? xr[k] : 0.0f;
+        if (k & 1) l1 = fmaf(xv[k], zr[k], l1);
+        else l0 = fmaf(xv[k], zr[k], l0);
+      }
+      float r, uu;
+      logistic_terms_f(l0 + l1, y[row], r, uu);
+      u += (double)uu;
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) gp[k] = fmaf(xv[k], r, gp[k]);
+    }
+    double pr = 0.0;
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) {
+      if (k < d) {
+        const float s = wave_sum(gp[k]);
+        if (lane == (k & (NAX_WAVE - 1))) g[k] = s + zr[k] * inv_s2;
+        pr += 0.5 * (double)zr[k] * zr[k] * inv_s2;
+      }
+    }
+    return wave_sum(u) + pr + log_norm;
+  }
+};
+
+template <int DMAX>
+struct LogisticChainPotential : Potential {
+  LogisticChainModel<DMAX> m;
+  double flops = 0.0, bytes = 0.0;
+  int dim() const override { return m.d; }
+  const char* name() const override { return "logistic_chain"; }
+  bool persistent() const override { return true; }
+  double alg_flops(int C) const override { return 4.0 * (double)m.n * m.d * C; }
+  double alg_bytes(int C) const override { return 4.0 * (double)m.n * (m.d + 1); }
+
+  hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe, float* g,
+                  int ldg) override {
+    mark_begin(s);
+    hipLaunchKernelGGL(perchain_eval_kernel<LogisticChainModel<DMAX>>, dim3((C + 3) / 4),
+                       dim3(256), 0, s, m, z, ldz, C, pe, g, ldg);
+    mark_end(s);
+    return hipGetLastError();
+  }
+  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                            int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
+                            float* gout, double* peout, int64_t* gevals) override {
+    mark_begin(s);
+    if (cfg.dense)
+      hipLaunchKernelGGL((persistent_kernel<LogisticChainModel<DMAX>, true, 1>), dim3(cfg.C),
+                         dim3(64), 0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout,
+                         gevals);
+    else
+      hipLaunchKernelGGL((persistent_kernel<LogisticChainModel<DMAX>, false, 1>), dim3(cfg.C),
+                         dim3(64), 0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout,
+                         gevals);
+    mark_end(s);
+    return hipGetLastError();
+  }
+};
+
+// d <= 64; x and y are the caller's device arrays (kept, not copied).
+Potential* make_logistic_chain(const float* x, const float* y, int64_t n, int d,
+                               double prior_scale, std::string* err) {
+  if (d < 1 || d > 64) {
+    *err = "the per-chain logistic kernel supports 1 <= dim <= 64";
+    return nullptr;
+  }
+  auto fill = [&](auto* p) {
+    p->m.x = x;
+    p->m.y = y;
+    p->m.n = n;
+    p->m.d = d;
+    p->m.inv_s2 = (float)(1.0 / (prior_scale * prior_scale));
+    p->m.log_norm = d * std::log(std::sqrt(2.0 * M_PI) * prior_scale);
+    return p;
+  };
+  if (d <= 32) return fill(new LogisticChainPotential<32>());
+  return fill(new LogisticChainPotential<64>());
+}
+
+}  // namespace nax

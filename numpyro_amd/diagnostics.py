@@ -1,8 +1,10 @@
 """Posterior diagnostics: ESS, (split) R-hat, HPDI, summary.
 
-Same formulas as ``numpyro/diagnostics.py`` (host NumPy, off the sampling
-path): ESS by FFT autocovariance and Geyer's initial monotone sequence
-(``diagnostics.py:101-203``), split Gelman-Rubin (``64-80``).
+Same formulas as ``numpyro/diagnostics.py``: ESS by autocovariance and
+Geyer's initial monotone sequence (``diagnostics.py:101-203``), split
+Gelman-Rubin (``64-80``).  Host NumPy for arrays; ``device_diagnostics``
+computes ESS and split R-hat of draws already on the GPU with the library's
+kernel (one workgroup per parameter).
 """
 import numpy as np
 
@@ -116,3 +118,26 @@ def print_summary(samples, prob=0.90, group_by_chain=True):
         mean = np.atleast_1d(st["mean"])
         print(f"{name}: mean[:4]={mean[:4]} min n_eff={np.min(st['n_eff']):.1f} "
               f"max r_hat={np.nanmax(np.atleast_1d(st['r_hat'])):.3f}")
+
+
+def device_diagnostics(x):
+    """(ESS, split R-hat) per parameter of draws ``x`` [chains, draws, ...] on
+    the GPU (float32 is read; results float64, on the GPU).  draws <= 4096."""
+    import torch
+
+    from . import _lib
+    x = torch.as_tensor(x)
+    if x.ndim < 2:
+        raise ValueError("x must be [chains, draws, ...]")
+    C, S = int(x.shape[0]), int(x.shape[1])
+    rest = tuple(x.shape[2:])
+    P = 1
+    for n in rest:
+        P *= int(n)
+    xf = x.to(device="cuda", dtype=torch.float32).reshape(C, S, P).contiguous()
+    ess = torch.empty(P, dtype=torch.float64, device="cuda")
+    rhat = torch.empty(P, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().nax_diagnostics(_lib.ptr(xf), C, S, P, _lib.ptr(ess), _lib.ptr(rhat)),
+               "nax_diagnostics")
+    return ess.reshape(rest), rhat.reshape(rest)

@@ -62,14 +62,14 @@ class LogisticRegression(Model):
         if x_dtype not in _lib.X_DTYPES:
             raise ValueError(f"unknown x_dtype {x_dtype!r}")
         self.x_dtype = x_dtype
-        if kernel not in ("auto", "tile", "mfma"):
-            raise ValueError("kernel must be 'auto', 'tile' or 'mfma'")
+        if kernel not in ("auto", "tile", "mfma", "chain"):
+            raise ValueError("kernel must be 'auto', 'tile', 'mfma' or 'chain'")
         self.kernel = kernel
         self.site_sizes = (("coefs", self.dim, None),)
 
     def attach(self, handle):
         lib = _lib.load()
-        kern = {"auto": 0, "tile": 1, "mfma": 2}[self.kernel]
+        kern = {"auto": 0, "tile": 1, "mfma": 2, "chain": 3}[self.kernel]
         _lib.check(lib.nax_model_logistic_ex(handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N,
                                              self.dim, self.prior_scale,
                                              _lib.X_DTYPES[self.x_dtype], kern),

@@ -127,6 +127,25 @@ def test_logistic_potential_grad_fp32(lib, N, D, C):
         np.testing.assert_allclose(g[c].cpu().numpy(), gr, rtol=1e-4, atol=1e-5 * scale)
 
 
+@pytest.mark.parametrize("N, D, C", [(1000, 26, 4), (1, 3, 1), (777, 55, 9), (100, 1, 3),
+                                     (5000, 64, 2), (63, 33, 5)])
+def test_logistic_potential_grad_chain_kernel(lib, N, D, C):
+    # one wavefront per chain (kernel "chain"): lane = row, d <= 64
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import LogisticRegression
+    X, y, Z = _logistic_problem(N, D, C, N + D + C + 1)
+    eng = Engine(LogisticRegression(X, y, prior_scale=0.9, kernel="chain"), C, "NUTS", 0)
+    z = torch.from_numpy(Z).cuda()
+    pe = torch.empty(C, dtype=torch.float64, device="cuda")
+    g = torch.empty((C, D), dtype=torch.float32, device="cuda")
+    lib.check(lib.load().nax_potential_grad(eng.h, lib.ptr(z), D, C, lib.ptr(pe), lib.ptr(g)))
+    for c in range(C):
+        u, gr = om.logistic_potential_grad(Z[c].astype(np.float64), X, y, 0.9)
+        np.testing.assert_allclose(pe[c].item(), u, rtol=1e-5)
+        scale = np.abs(gr).max() + 1.0
+        np.testing.assert_allclose(g[c].cpu().numpy(), gr, rtol=1e-4, atol=1e-5 * scale)
+
+
 def test_leapfrog_trajectory_matches_oracle(lib):
     # L-step velocity_verlet from identical (q, p, eps, M): 1e-5 relative
     N, D, C, L = 1000, 26, 4, 25

@@ -34,7 +34,7 @@ def _run_engine(X, y, z0, keys, num_warmup, num_samples, algo="NUTS", model_kw=N
     from numpyro_amd.infer import hmc
     from numpyro_amd.infer.util import fori_collect
     from numpyro_amd.models import LogisticRegression
-    m = LogisticRegression(X, y, **(model_kw or {}))
+    m = LogisticRegression(X, y, **(model_kw or {"kernel": "tile"}))
     init_kernel, sample_kernel = hmc(m, algo=algo)
     st = init_kernel(torch.from_numpy(z0), num_warmup, rng_key=keys, **kw)
     out = fori_collect(0, num_warmup + num_samples, sample_kernel, st,
@@ -97,7 +97,7 @@ def _step_sizes(run, X, y, z0, keys, num_warmup, num_samples, **kw):
     if run == "engine":
         from numpyro_amd.infer import hmc
         from numpyro_amd.models import LogisticRegression
-        init_kernel, sample_kernel = hmc(LogisticRegression(X, y), algo="NUTS")
+        init_kernel, sample_kernel = hmc(LogisticRegression(X, y, kernel="tile"), algo="NUTS")
         st = init_kernel(torch.from_numpy(z0), num_warmup, rng_key=keys, **kw)
         rows = [torch.as_tensor(st.adapt_state.step_size).cpu().numpy().reshape(-1)]
         ns = []
@@ -161,7 +161,7 @@ def _run_pair(X, y, z0, keys, num_warmup, num_samples, **kw):
     and the final adapt_state (inverse mass matrix, mass_matrix_sqrt)."""
     from numpyro_amd.infer import hmc
     from numpyro_amd.models import LogisticRegression
-    init_kernel, sample_kernel = hmc(LogisticRegression(X, y), algo="NUTS")
+    init_kernel, sample_kernel = hmc(LogisticRegression(X, y, kernel="tile"), algo="NUTS")
     st = init_kernel(torch.from_numpy(z0), num_warmup, rng_key=keys, **kw)
     gz, gns = [], []
     for _ in range(num_warmup + num_samples):
@@ -247,3 +247,33 @@ def test_compacted_mfma_chains_match_oracle():
     assert np.array_equal(gns, ons), np.argwhere(gns != ons)[:5]
     np.testing.assert_allclose(gz, oz, rtol=1e-3, atol=1e-3)
     np.testing.assert_allclose(gpe, ope, rtol=1e-5)
+
+
+def test_chain_kernel_matches_oracle():
+    # one wavefront per chain, whole loop in one persistent launch (the
+    # "auto" choice for a handful of chains on a small design)
+    N, D, C = 300, 6, 4
+    X, y = _problem(N, D, 61)
+    z0 = np.random.default_rng(14).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(37), C)[c] for c in range(C)])
+    kw = dict(adapt_step_size=False, adapt_mass_matrix=False, step_size=0.2)
+    gz, gns, gpe, gap = _run_engine(X, y, z0, keys, 0, 40, model_kw=dict(kernel="chain"), **kw)
+    oz, ons, ope, oap = _run_oracle(X, y, z0, keys, 0, 40, **kw)
+    assert np.array_equal(gns, ons), (gns, ons)
+    np.testing.assert_allclose(gz, oz, rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(gpe, ope, rtol=1e-5)
+    np.testing.assert_allclose(gap, oap, rtol=1e-3, atol=1e-4)
+
+
+def test_chain_kernel_adapted_matches_oracle():
+    N, D, C = 300, 6, 4
+    X, y = _problem(N, D, 62)
+    z0 = np.random.default_rng(15).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(41), C)[c] for c in range(C)])
+    gz, gns, gpe, _ = _run_engine(X, y, z0, keys, 40, 20, model_kw=dict(kernel="chain"))
+    oz, ons, ope, _ = _run_oracle(X, y, z0, keys, 40, 20)
+    # adapted chains agree until amplified float32 rounding flips a near tie
+    for c in range(C):
+        bad = np.flatnonzero(gns[c] != ons[c])
+        assert bad.size == 0 or bad[0] >= 25, (c, gns[c], ons[c])
+    np.testing.assert_allclose(gz[:, :15], oz[:, :15], rtol=1e-3, atol=1e-3)
