@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
     ap.add_argument("--chains", type=int, default=None)
-    ap.add_argument("--fp32-steps", type=int, default=8,
+    ap.add_argument("--fp32-steps", type=int, default=24,
                     help="timed iterations of the plain FP32 form (0: skip)")
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of CPU-baseline work (0: skip)")
@@ -163,7 +163,12 @@ def main():
         e32, o32, w32, _, k32, n32 = run_form(m32, C, args.warmup, args.fp32_steps, z0, keys,
                                               state=post_state)
         me32, rh32 = min_ess(o32["z"]) if args.fp32_steps >= 4 else (float("nan"), float("nan"))
+        # the bf16 chains' ESS over the same number of draws, for comparison
+        mes_b, _ = min_ess(out["z"][:, :args.fp32_steps])
         fp32 = {"grad_evals_per_s": e32.grad_evals / w32, "min_ess_per_s": me32 / w32,
+                "min_ess_per_iteration": me32 / args.fp32_steps,
+                "bf16_min_ess_per_iteration_same_draws": mes_b / args.fp32_steps,
+                "max_rhat": rh32,
                 "steps": args.fp32_steps, "kernel_avg_ms": k32 / max(n32, 1),
                 "mean_tree": float(o32["num_steps"].float().mean())}
         log(f"fp32 form: {fp32}")
@@ -203,6 +208,7 @@ def main():
                    "parallelism": "chains"},
         "min_ess_per_s": mess / wall,
         "min_ess": mess,
+        "min_ess_per_iteration": mess / args.steps,
         "max_rhat": rhat,
         "mean_tree_size": float(ns.mean()),
         "mean_accept_prob": acc,
