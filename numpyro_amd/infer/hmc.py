@@ -406,6 +406,7 @@ class HMC(MCMCKernel):
         target = self._target_model()
         keys = np.asarray(rng_key, dtype=np.uint32).reshape(-1, 2)
         num_chains = keys.shape[0]
+        user_params = init_params is not None
         if init_params is None:
             if not isinstance(target, Model):
                 raise ValueError(
@@ -425,7 +426,13 @@ class HMC(MCMCKernel):
                      max_tree_depth=self._max_tree_depth,
                      find_heuristic_step_size=self._find_heuristic_step_size,
                      regularize_mass_matrix=self._regularize_mass_matrix)
-        eng.init(z0.reshape(num_chains, -1), init_keys, self._inverse_mass_matrix)
+        z0 = z0.reshape(num_chains, -1)
+        if not getattr(model, "valid_obs", True):
+            raise RuntimeError("Cannot find valid initial parameters. Please check your model "
+                               "again.")
+        if not user_params and not isinstance(model, PotentialFn):
+            z0 = _find_valid_initial_params(model, z0, keys, eng)
+        eng.init(z0, init_keys, self._inverse_mass_matrix)
         self._engine = eng
         if isinstance(model, Model) and not isinstance(model, PotentialFn):
             self._postprocess_fn = model.postprocess
@@ -477,6 +484,32 @@ def _split_first(keys):
     for c in range(keys.shape[0]):
         out[c] = nrandom.split(keys[c])[0]
     return out
+
+
+def _find_valid_initial_params(model, z0, keys, eng, max_tries=100):
+    """``find_valid_initial_params`` (``numpyro/infer/util.py:366-508``):
+    redraw a chain's initial point (uniform(-2, 2) from a fresh split of its
+    key) while U or dU/dz is not finite there, at most 100 times."""
+    z = z0.clone()
+    keys = np.array(keys, dtype=np.uint32).reshape(-1, 2)
+    C_, D = z.shape
+    ld = _lib.stride(eng.h)
+    lib = _lib.load()
+    for _ in range(max_tries):
+        zp = torch.zeros((C_, ld), dtype=torch.float32, device="cuda")
+        zp[:, :D] = z.to("cuda")
+        u = torch.empty(C_, dtype=torch.float64, device="cuda")
+        g = torch.zeros((C_, ld), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(lib.nax_potential_grad(eng.h, _lib.ptr(zp), ld, C_, _lib.ptr(u), _lib.ptr(g)),
+                   "nax_potential_grad")
+        bad = (~(torch.isfinite(u) & torch.isfinite(g[:, :D]).all(dim=1))).cpu()
+        if not bool(bad.any()):
+            return z
+        for c in torch.nonzero(bad).flatten().tolist():
+            keys[c] = nrandom.split(keys[c])[0]
+            z[c] = init_to_uniform(keys[c:c + 1], z.shape[1], self_split=True)[0]
+    raise RuntimeError("Cannot find valid initial parameters. Please check your model again.")
 
 
 def init_to_uniform(keys, dim, radius=2.0, self_split=False):

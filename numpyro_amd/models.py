@@ -6,10 +6,23 @@ and ``postprocess_fn`` (constrain, ``infer/util.py:663-835``).  Latents are a
 flat vector in sorted site-name order, as ``ravel_pytree`` flattens a dict.
 """
 import math
+import warnings
 
 import torch
 
 from . import _lib
+
+
+def _check_binary_obs(y):
+    """BernoulliLogits' support is {0, 1}; with validation on (numpyro's
+    default, ``distributions/distribution.py:67, 430-440``) other observed
+    values warn and give log_prob = -inf, so no initial point is valid.
+    Returns True when every label is 0 or 1."""
+    ok = bool(((y == 0) | (y == 1)).all())
+    if not ok:
+        warnings.warn("Out-of-support values provided to log prob method. "
+                      "The value argument should be within the support.", stacklevel=3)
+    return ok
 
 
 class Model:
@@ -58,6 +71,7 @@ class LogisticRegression(Model):
         self.X = X.to(device="cuda", dtype=torch.float32).contiguous()
         self.y = y.to(device="cuda", dtype=torch.float32).contiguous()
         self.N, self.dim = self.X.shape
+        self.valid_obs = _check_binary_obs(self.y)
         self.prior_scale = float(prior_scale)
         if x_dtype not in _lib.X_DTYPES:
             raise ValueError(f"unknown x_dtype {x_dtype!r}")
@@ -148,6 +162,7 @@ class HierarchicalLogistic(Model):
         self.dim = max(self.dims) + 2
         self.X = torch.cat([x.reshape(-1) for x in xs]).to("cuda").contiguous()
         self.y = torch.cat(yv).to("cuda").contiguous()
+        self.valid_obs = _check_binary_obs(self.y)
         elt = [0]
         for n, d in zip(self.rows, self.dims):
             elt.append(elt[-1] + n * d)

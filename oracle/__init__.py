@@ -24,4 +24,10 @@ pinned by at least one of: a reference known-answer test re-expressed in
 ``tests/test_oracle_*.py``, a published known-answer vector, a closed form,
 scipy, or central finite differences.  Functions without such a pin say
 "parity unpinned" in their docstring.
+
+Parity unpinned against jax itself (jax is absent and was never imported):
+``jax_random.split`` / ``random_bits`` in the partitionable mode (the
+original mode is pinned by the documented ``split(PRNGKey(0))`` value), and
+the float32 bits of ``jax_random.normal`` (pinned only to the mathematical
+sqrt(2) * erfinv(u), which XLA's float32 erf_inv approximates).
 """

@@ -69,7 +69,8 @@ def PRNGKey(seed):
 
 
 def split(key, num=2, partitionable=None):
-    """``random.split(key, num)`` -> uint32 array [num, 2]."""
+    """``random.split(key, num)`` -> uint32 array [num, 2].  Partitionable
+    mode: parity unpinned against jax (see the module docstring)."""
     part = PARTITIONABLE if partitionable is None else partitionable
     key = np.asarray(key, dtype=np.uint32)
     if part:
@@ -115,7 +116,8 @@ _NORMAL_LO = np.nextafter(np.float32(-1.0), np.float32(0.0))
 
 def normal(key, n, partitionable=None):
     """``random.normal(key, (n,))`` = sqrt(2) * erfinv(u), u ~ U(nextafter(-1,0), 1)
-    in float32; erfinv evaluated in float64 (the mathematical value)."""
+    in float32; erfinv evaluated in float64 (the mathematical value).  The
+    float32 bits of XLA's erf_inv: parity unpinned."""
     u = uniform(key, n, _NORMAL_LO, 1.0, partitionable)
     return np.sqrt(2.0) * erfinv(u.astype(np.float64))
 

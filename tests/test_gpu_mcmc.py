@@ -143,3 +143,18 @@ def test_graph_replay_matches_direct_launches():
     for k in outs[0]:
         assert np.array_equal(outs[0][k], outs[1][k]), k
     assert outs[0]["num_steps"].sum() > 16 * 3 * C  # enough ticks for several replays
+
+
+def test_out_of_support_obs_cannot_initialise():
+    # BernoulliLogits with labels outside {0, 1}: numpyro warns and
+    # find_valid_initial_params fails (infer/util.py:366-508, 830-832)
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.infer import MCMC, NUTS
+    from numpyro_amd.models import LogisticRegression
+    X = np.random.default_rng(0).normal(size=(50, 3)).astype(np.float32)
+    y = np.full(50, 0.5, np.float32)
+    with pytest.warns(UserWarning, match="Out-of-support"):
+        m = LogisticRegression(X, y)
+    mcmc = MCMC(NUTS(m), num_warmup=5, num_samples=5)
+    with pytest.raises(RuntimeError, match="Cannot find valid initial parameters"):
+        mcmc.run(nrandom.PRNGKey(0))
