@@ -310,13 +310,37 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
 // ---- pipelined BF16 kernel --------------------------------------------------------
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
 
+// One 16-byte global->LDS load per lane (global_load_lds_dwordx4): lane l
+// writes LDS byte lds_base + 16 l (lds_base wave-uniform, in M0).  Issued as
+// inline assembly so that the compiler does not tie later LDS reads to it:
+// given the builtin, the waitcnt pass makes every later ds_read of the block
+// wait for all outstanding LDS-DMA loads (vmcnt(0)), which turns a prefetch
+// several tiles ahead into a wait on the tile just issued.  The kernel waits
+// for these loads itself with counted s_waitcnt vmcnt + a barrier.
+__device__ __forceinline__ void lds_dma16(const void* gptr, uint32_t lds_base) {
+  asm volatile(
+      "s_mov_b32 m0, %1\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %0, off"
+      :
+      : "v"(gptr), "s"(lds_base)
+      : "memory", "m0");
+}
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
 // Software-pipelined form of logit_mfma for bf16 X.  The block's row tiles
-// i = 0..n-1 (tile rb + i RB) rotate through a 4-slot LDS ring filled by
-// direct global->LDS loads (no staging registers) issued two tiles ahead.
-// Iteration i runs, as one branch-free basic block, the epilogue of tile i
-// (VALU) interleaved in source order with GEMM1 of tile i+1 and GEMM2 of
-// tile i-1 (MFMA), so the matrix cores and the VALU work on different tiles
-// at once.  GEMM2 runs in fp16: X (bf16 values) is exact in fp16 and R gets
+// i = 0..n-1 (tile rb + i RB) stream through LDS by direct global->LDS loads
+// (no staging registers).  Iteration i runs, as one branch-free basic block,
+// the epilogue of tile i (VALU) interleaved in source order with GEMM1 of
+// tile i+1 and GEMM2 of tile i-1 (MFMA), so the matrix cores and the VALU
+// work on different tiles at once.  The three parts of a tile live for
+// different iterations (A1 of tile t is read in iteration t-1, y in t, B2 in
+// t+1), so each has its own 4-slot ring: iteration i issues A1 of tile i+4,
+// B2 of tile i+2 and y of tile i+3 into the slots freed by iteration i-1,
+// and then waits only for the loads issued two iterations earlier -- every
+// load has three iterations (~4 us) to arrive.  GEMM2 runs in fp16: X (bf16 values) is exact in fp16 and R gets
 // 11 mantissa bits in a single MFMA.
 template <int DP>
 __global__ void __launch_bounds__(256, 1)
@@ -330,14 +354,19 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   constexpr int ABYTES = KK * 64 * 16;
   constexpr int BBYTES = 2 * DT * 64 * 16;
   constexpr int TBYTES = ABYTES + BBYTES + 128;
-  constexpr int PF = (ABYTES + BBYTES) / (256 * 16);
-  constexpr int PA = ABYTES / 16;
-  static_assert((ABYTES + BBYTES) % (256 * 16) == 0, "tile must split into 16-byte chunks");
-  static_assert(PA % 64 == 0, "a wave's 64 chunks must not straddle A1 and B2");
-  __shared__ __attribute__((aligned(16))) char lds[4][TBYTES];
+  // A1 and B2 parts of a tile are KK wave-loads of 64 16-byte chunks each;
+  // wave w issues the wave-loads w, w + 4, ... of each part.
+  constexpr int PP = (KK + 3) / 4;
+  static_assert(ABYTES == KK * 64 * 16 && BBYTES == KK * 64 * 16, "part sizes");
+  // loads per iteration of the wave that issues the fewest (wave 3)
+  constexpr int NMIN = 2 * (KK / 4) + 1;
+  (void)TBYTES;
+  __shared__ __attribute__((aligned(16))) char lds_a[4][ABYTES];
+  __shared__ __attribute__((aligned(16))) char lds_b[4][BBYTES];
+  __shared__ __attribute__((aligned(16))) char lds_y[4][128];
 
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l = tid & 63;
   const int cl = l & 31, h = l >> 5;
   // Chains are taken through the compacted list act[0..*nact) of the chains
@@ -383,34 +412,51 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   double u = 0.0;
 
   const int64_t n = rb < ntile ? (ntile - 1 - rb) / RB + 1 : 0;
-  // Tile i (clamped to the block's last tile, so always in bounds) goes to
-  // slot i & 3.  Lane l of wave w loads chunk tid + 256 p to LDS offset
-  // 16 (tid + 256 p): each wave's 64 chunks are contiguous, as the
-  // LDS-direct load (LDS address = wave base + 16 lane) requires.
-  auto load_tile = [&](int64_t i) {
-    const int64_t t = rb + (i < n ? i : n - 1) * (int64_t)RB;
-    const uint4* ga = (const uint4*)(A1 + (size_t)t * KK * 64);
-    const uint4* gb = (const uint4*)(B2 + (size_t)t * 2 * DT * 64);
-    char* base = lds[i & 3];
+  // Tile i is clamped to the block's last tile, so every load is in bounds
+  // and a wave issues the same number of loads in every iteration, which
+  // the counted waits below rely on.  Lane l of wave w loads chunk
+  // tid + 256 p to LDS offset 16 (tid + 256 p): each wave's 64 chunks are
+  // contiguous, as the LDS-direct load (LDS address = wave base + 16 lane)
+  // requires.  The 32 labels of a tile are 8 chunks: lanes 0-1 of each wave.
+  auto tile_of = [&](int64_t i) { return rb + (i < n ? i : n - 1) * (int64_t)RB; };
+  auto issue_a = [&](int64_t i) {
+    const uint4* ga = (const uint4*)(A1 + (size_t)tile_of(i) * KK * 64);
+    char* base = lds_a[i & 3];
 #pragma unroll
-    for (int p = 0; p < PF; ++p) {
-      const int idx = tid + 256 * p;
-      const uint4* src = idx < PA ? ga + idx : gb + (idx - PA);
-      __builtin_amdgcn_global_load_lds(
-          (const void*)src,
-          (__attribute__((address_space(3))) void*)(base + 16 * (256 * p + 64 * wave)), 16, 0, 0);
-    }
-    if (tid < 8)
-      __builtin_amdgcn_global_load_lds(
-          (const void*)(ypad + 32 * t + 4 * tid),
-          (__attribute__((address_space(3))) void*)(base + ABYTES + BBYTES), 16, 0, 0);
+    for (int p = 0; p < PP; ++p)
+      if (KK % 4 == 0 || 4 * p + wave < KK)
+        lds_dma16(ga + tid + 256 * p, lds_addr(base + 16 * (256 * p + 64 * wave)));
   };
-  auto wait_tiles = [&]() {
+  auto issue_b = [&](int64_t i) {
+    const uint4* gb = (const uint4*)(B2 + (size_t)tile_of(i) * 2 * DT * 64);
+    char* base = lds_b[i & 3];
+#pragma unroll
+    for (int p = 0; p < PP; ++p)
+      if (KK % 4 == 0 || 4 * p + wave < KK)
+        lds_dma16(gb + tid + 256 * p, lds_addr(base + 16 * (256 * p + 64 * wave)));
+  };
+  auto issue_y = [&](int64_t i) {
+    if (l < 2) lds_dma16(ypad + 32 * tile_of(i) + 4 * (2 * wave + l), lds_addr(lds_y[i & 3] + 32 * wave));
+  };
+  auto issue_group = [&](int64_t i) {
+    issue_a(i + 4);
+    issue_b(i + 2);
+    issue_y(i + 3);
+  };
+  // After iteration i only the groups of iterations i-1 and i may still be
+  // in flight (at most 2 NMIN loads of any wave, loads retire in order); the
+  // barrier then publishes every wave's loads to the block.
+  static_assert(2 * NMIN < 64, "vmcnt holds 6 bits");
+  auto wait_group = [&]() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * NMIN) : "memory");
+    __syncthreads();
+  };
+  auto wait_all = [&]() {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
   };
   auto gemm1 = [&](int64_t i) {
-    const bf16x8* la = (const bf16x8*)lds[i & 3];
+    const bf16x8* la = (const bf16x8*)lds_a[i & 3];
     f32x16 L;
 #pragma unroll
     for (int v = 0; v < 16; ++v) L[v] = 0.0f;
@@ -423,7 +469,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     return L;
   };
   auto gemm2 = [&](int64_t i, f16x8 r0, f16x8 r1) {
-    const f16x8* lb = (const f16x8*)(lds[i & 3] + ABYTES);
+    const f16x8* lb = (const f16x8*)lds_b[i & 3];
 #pragma unroll
     for (int dt = 0; dt < DT; ++dt) {
       g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(r0, lb[dt * 64 + l], g[dt], 0, 0, 0);
@@ -435,7 +481,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   // written.  No per-element masking is needed.  Register v of L holds row
   // 8(v/4) + 4h + v%4.
   auto load_y = [&](int64_t i, float* yv) {
-    const float* ly = (const float*)(lds[i & 3] + ABYTES + BBYTES);
+    const float* ly = (const float*)lds_y[i & 3];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float4 yq = *(const float4*)(ly + 8 * q + 4 * h);
@@ -463,8 +509,8 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   // element by element in source order.
   auto fused_step = [&](int64_t i, const f32x16& Lc, f16x8 rp0, f16x8 rp1, f16x8& r0,
                         f16x8& r1) {
-    const bf16x8* la = (const bf16x8*)lds[(i + 1) & 3];
-    const f16x8* lb = (const f16x8*)(lds[(i - 1) & 3] + ABYTES);
+    const bf16x8* la = (const bf16x8*)lds_a[(i + 1) & 3];
+    const f16x8* lb = (const f16x8*)lds_b[(i - 1) & 3];
     float yv[16];
     load_y(i, yv);
     f32x16 Ln;
@@ -496,8 +542,8 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   };
   auto fused_step256 = [&](int64_t i, const f32x16& Lc, f16x8 rp0, f16x8 rp1, f16x8& r0,
                         f16x8& r1) {
-    const bf16x8* la = (const bf16x8*)lds[(i + 1) & 3];
-    const f16x8* lb = (const f16x8*)(lds[(i - 1) & 3] + ABYTES);
+    const bf16x8* la = (const bf16x8*)lds_a[(i + 1) & 3];
+    const f16x8* lb = (const f16x8*)lds_b[(i - 1) & 3];
     float yv[16];
     load_y(i, yv);
     f32x16 Ln;
@@ -542,28 +588,38 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   };
 
   if (n > 0) {
-    load_tile(0);
-    load_tile(1);
-    wait_tiles();
-    load_tile(2);
+    // the groups of the virtual iterations -4..-1: A1 of tiles 0-3, B2 of
+    // tiles 0-1, y of tiles 0-2
+    issue_a(0);
+    issue_a(1);
+    issue_y(0);
+    issue_a(2);
+    issue_b(0);
+    issue_y(1);
+    issue_a(3);
+    issue_b(1);
+    issue_y(2);
+    wait_all();
     f32x16 Lc = gemm1(0);
     f16x8 rp0, rp1;
     {  // i = 0: epilogue(0) + gemm1(1)
+      __syncthreads();  // A1 slot 0 is refilled next: every wave has read tile 0
+      issue_group(0);
       f16x8 r0, r1;
       epilogue(0, Lc, r0, r1);
       if (n > 1) Lc = gemm1(1);
       rp0 = r0;
       rp1 = r1;
-      wait_tiles();
+      wait_group();
     }
     for (int64_t i = 1; i + 1 < n; ++i) {  // steady state, branch-free body
-      load_tile(i + 2);
+      issue_group(i);
       f16x8 r0, r1;
       if constexpr (DP == 256) Lc = fused_step256(i, Lc, rp0, rp1, r0, r1);
       else Lc = fused_step(i, Lc, rp0, rp1, r0, r1);
       rp0 = r0;
       rp1 = r1;
-      wait_tiles();
+      wait_group();
     }
     if (n > 1) {  // i = n-1: epilogue(n-1) + gemm2(n-2)
       f16x8 r0, r1;
@@ -572,6 +628,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       rp0 = r0;
       rp1 = r1;
     }
+    wait_all();  // B2 of tile n-1 (and the clamped tail loads) have landed
     gemm2(n - 1, rp0, rp1);
   }
 
