@@ -127,6 +127,14 @@ int nax_model_logistic(nax_sampler* h, const float* d_x, const float* d_y, int64
  * per call (one wavefront per chain). */
 int nax_model_sv(nax_sampler* h, const float* d_returns, int32_t T, int32_t noncentred);
 
+/* nax_model_sv with the two prior rates as arguments: sigma ~
+ * Exponential(sigma_rate), nu ~ Exponential(nu_rate) (both finite and > 0,
+ * else NAX_EINVAL).  The target of the model compiler for stochastic-
+ * volatility models written with other rates (the reference builds the same
+ * potential from the model function, numpyro/infer/util.py:300-358). */
+int nax_model_sv_ex(nax_sampler* h, const float* d_returns, int32_t T, int32_t noncentred,
+                    double sigma_rate, double nu_rate);
+
 /* Built-in potential: `groups` independent non-centred hierarchical logistic
  * regressions (eta ~ Normal(0,1)^D_i, mu ~ Normal(0,1), tau ~ HalfCauchy(1),
  * beta = mu + tau eta, y ~ Bernoulli(logits = X_i beta)), chains_per_group

@@ -60,6 +60,8 @@ SIGNATURES = {
     "nax_model_logistic_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                         C.c_int32, C.c_double, C.c_int32, C.c_int32]),
     "nax_model_sv": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
+    "nax_model_sv_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                  C.c_double]),
     "nax_model_hier": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_int32, C.c_int32]),
     "nax_model_external": (C.c_int, [C.c_void_p]),

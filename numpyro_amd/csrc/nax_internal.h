@@ -123,7 +123,8 @@ Potential* make_logistic_chain(const float* x, const float* y, int64_t n, int d,
 Potential* make_hier(const float* x, const float* y, const int64_t* row_off, const int64_t* elt_off,
                      const int* dims, int groups, int chains_per_group, int num_chains, int Dp,
                      hipStream_t s, std::string* err);
-Potential* make_sv(const float* returns, int T, int noncentred, hipStream_t s, std::string* err);
+Potential* make_sv(const float* returns, int T, int noncentred, double sigma_rate, double nu_rate,
+                   hipStream_t s, std::string* err);
 
 void set_error(const std::string& msg);
 

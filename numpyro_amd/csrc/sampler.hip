@@ -393,17 +393,22 @@ extern "C" int nax_model_logistic_ex(nax_sampler* h, const float* x, const float
   return NAX_OK;
 }
 
-extern "C" int nax_model_sv(nax_sampler* h, const float* returns, int32_t T, int32_t noncentred) {
+extern "C" int nax_model_sv_ex(nax_sampler* h, const float* returns, int32_t T,
+                               int32_t noncentred, double sigma_rate, double nu_rate) {
   if (!h || !returns) { set_error("nax_model_sv: null argument"); return NAX_EINVAL; }
   if (T + 2 != h->cfg.D) { set_error("sv model: dim must be T + 2"); return NAX_EINVAL; }
   std::string err;
-  Potential* p = make_sv(returns, T, noncentred, h->stream, &err);
+  Potential* p = make_sv(returns, T, noncentred, sigma_rate, nu_rate, h->stream, &err);
   if (!p) { set_error(err); return NAX_EINVAL; }
   delete h->pot;
   h->pot = p;
   h->external = false;
   h->cfg.chain_dim = nullptr;
   return NAX_OK;
+}
+
+extern "C" int nax_model_sv(nax_sampler* h, const float* returns, int32_t T, int32_t noncentred) {
+  return nax_model_sv_ex(h, returns, T, noncentred, 50.0, 0.1);
 }
 
 extern "C" int nax_model_hier(nax_sampler* h, const float* x, const float* y,

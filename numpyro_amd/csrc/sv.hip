@@ -1,6 +1,7 @@
 // Stochastic-volatility potential and its hand-written adjoint, one wavefront
-// per chain (examples/stochastic_volatility.py:58-66):
-//   sigma ~ Exponential(50), nu ~ Exponential(0.1),
+// per chain (examples/stochastic_volatility.py:58-66, whose rates are
+// a = 50, b = 0.1):
+//   sigma ~ Exponential(a), nu ~ Exponential(b),
 //   s ~ GaussianRandomWalk(sigma, T), r_t ~ StudentT(nu, 0, exp(s_t)).
 // Log-densities: Exponential continuous.py:715-726, GaussianRandomWalk
 // continuous.py:1104-1112, StudentT continuous.py:3229-3238; positive sites
@@ -49,8 +50,6 @@ __device__ __forceinline__ double digamma_pos(double x) {
   return acc + log(x) - 0.5 / x - series;
 }
 
-constexpr double kSigmaRate = 50.0;
-constexpr double kNuRate = 0.1;
 constexpr double kLogPi = 1.1447298858494002;
 constexpr double kLog2Pi = 1.8378770664093453;
 
@@ -58,6 +57,8 @@ struct SVModel {
   const float* r;  // returns [T]
   int T;
   int noncentred;
+  double sigma_rate;  // sigma ~ Exponential(sigma_rate)
+  double nu_rate;     // nu ~ Exponential(nu_rate)
 
   // T-dependent constant part of U and of dU/dnu for the StudentT block.
   __device__ void studentt_consts(double nu, double& u_const, double& dnu_const) const {
@@ -96,11 +97,11 @@ struct SVModel {
     double uc, dnc;
     studentt_consts(nu, uc, dnc);
     U += uc + T * (u_sig + 0.5 * kLog2Pi)             // GRW normaliser
-         - log(kSigmaRate) + kSigmaRate * sig - u_sig    // Exponential(50) + log|J|
-         - log(kNuRate) + kNuRate * nu - u_nu;           // Exponential(0.1) + log|J|
+         - log(sigma_rate) + sigma_rate * sig - u_sig    // Exponential(a) + log|J|
+         - log(nu_rate) + nu_rate * nu - u_nu;           // Exponential(b) + log|J|
     if (lane == 0) {
-      g[0] = (float)(nu * (dnu + dnc + kNuRate) - 1.0);
-      g[T + 1] = (float)(-S2 / (sig * sig) + T + kSigmaRate * sig - 1.0);
+      g[0] = (float)(nu * (dnu + dnc + nu_rate) - 1.0);
+      g[T + 1] = (float)(-S2 / (sig * sig) + T + sigma_rate * sig - 1.0);
     }
     return U;
   }
@@ -160,11 +161,11 @@ struct SVModel {
     double uc, dnc;
     studentt_consts(nu, uc, dnc);
     U += uc + T * 0.5 * kLog2Pi                         // N(0,1) normaliser of zeta
-         - log(kSigmaRate) + kSigmaRate * sig - u_sig
-         - log(kNuRate) + kNuRate * nu - u_nu;
+         - log(sigma_rate) + sigma_rate * sig - u_sig
+         - log(nu_rate) + nu_rate * nu - u_nu;
     if (lane == 0) {
-      g[0] = (float)(nu * (dnu + dnc + kNuRate) - 1.0);
-      g[1] = (float)(sig * asc + kSigmaRate * sig - 1.0);
+      g[0] = (float)(nu * (dnu + dnc + nu_rate) - 1.0);
+      g[1] = (float)(sig * asc + sigma_rate * sig - 1.0);
     }
     return U;
   }
@@ -203,9 +204,15 @@ struct SVPotential : Potential {
   }
 };
 
-Potential* make_sv(const float* returns, int T, int noncentred, hipStream_t s, std::string* err) {
+Potential* make_sv(const float* returns, int T, int noncentred, double sigma_rate, double nu_rate,
+                   hipStream_t s, std::string* err) {
   if (T < 2) {
     *err = "the stochastic-volatility model needs T >= 2";
+    return nullptr;
+  }
+  if (!(sigma_rate > 0.0) || !(nu_rate > 0.0) || !std::isfinite(sigma_rate) ||
+      !std::isfinite(nu_rate)) {
+    *err = "the stochastic-volatility model needs finite positive Exponential rates";
     return nullptr;
   }
   auto* p = new SVPotential();
@@ -221,6 +228,8 @@ Potential* make_sv(const float* returns, int T, int noncentred, hipStream_t s, s
   p->m.r = p->r;
   p->m.T = T;
   p->m.noncentred = noncentred;
+  p->m.sigma_rate = sigma_rate;
+  p->m.nu_rate = nu_rate;
   return p;
 }
 

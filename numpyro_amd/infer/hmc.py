@@ -18,6 +18,7 @@ import torch
 
 from .. import _lib
 from .. import random as nrandom
+from ..compile import compile_model, is_model_function
 from ..models import Model, PotentialFn
 from .mcmc import MCMCKernel
 
@@ -379,6 +380,7 @@ class HMC(MCMCKernel):
         self._regularize_mass_matrix = regularize_mass_matrix
         self._engine = None
         self._postprocess_fn = None
+        self._deterministic_sites = ()
 
     @property
     def model(self):
@@ -406,6 +408,14 @@ class HMC(MCMCKernel):
         target = self._target_model()
         keys = np.asarray(rng_key, dtype=np.uint32).reshape(-1, 2)
         num_chains = keys.shape[0]
+        if self._model is not None and is_model_function(self._model):
+            # NUTS(model): trace the model once and bind the matching kernels
+            # (the reference's initialize_model, numpyro/infer/util.py:663-835)
+            target = compile_model(self._model, *model_args, **(model_kwargs or {}))
+            target.bind(num_chains)
+            self._deterministic_sites = tuple(target.deterministic)
+            if isinstance(init_params, dict):
+                init_params = target.flatten(init_params)
         user_params = init_params is not None
         if init_params is None:
             if not isinstance(target, Model):

@@ -123,7 +123,7 @@ class MCMC:
             else:
                 collect_fields[f] = None
         collect_fields = tuple(collect_fields)
-        if init_params is not None and self.num_chains > 1:
+        if init_params is not None and self.num_chains > 1 and not isinstance(init_params, dict):
             if torch.as_tensor(init_params).shape[0] != self.num_chains:
                 raise ValueError("`init_params` must have the same leading dimension"
                                  " as `num_chains`.")
@@ -137,7 +137,8 @@ class MCMC:
             eng = self.sampler._engine
             eng.set_state(self._warmup_state, rng_key=chain_keys)
         else:
-            if init_params is not None and self.num_chains == 1:
+            if init_params is not None and self.num_chains == 1 and \
+                    not isinstance(init_params, dict):
                 init_params = torch.as_tensor(init_params).reshape(1, -1)
             self.sampler.init(chain_keys, self.num_warmup, init_params, args, kwargs)
             eng = self.sampler._engine
@@ -190,7 +191,9 @@ class MCMC:
         if torch.is_tensor(sites):
             sites = sites.cpu().numpy()
         elif isinstance(sites, dict):
-            sites = {k: v.cpu().numpy() for k, v in sites.items()}
+            skip = set(getattr(self.sampler, "_deterministic_sites", ())) \
+                if exclude_deterministic else set()
+            sites = {k: v.cpu().numpy() for k, v in sites.items() if k not in skip}
         print_summary(sites, prob=prob)
         ex = self.get_extra_fields()
         if "diverging" in ex:

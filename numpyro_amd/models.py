@@ -107,12 +107,13 @@ def _canon(x_dtype):
 
 class StochasticVolatility(Model):
     """``examples/stochastic_volatility.py:58-66``: sigma ~ Exponential(50),
-    s ~ GaussianRandomWalk(sigma, T), nu ~ Exponential(0.1),
+    s ~ GaussianRandomWalk(sigma, T), nu ~ Exponential(0.1) (other rates via
+    ``sigma_rate`` / ``nu_rate``),
     r ~ StudentT(nu, 0, exp(s)).  Latent (sorted site names, positive sites
     through ExpTransform): ``[log nu, s (T), log sigma]``; with
     ``noncentred=True`` ``[log nu, log sigma, zeta (T)]``, s = sigma cumsum(zeta)."""
 
-    def __init__(self, returns, noncentred=False):
+    def __init__(self, returns, noncentred=False, sigma_rate=50.0, nu_rate=0.1):
         r = torch.as_tensor(returns)
         if r.ndim != 1:
             raise ValueError("returns must be a vector")
@@ -120,14 +121,19 @@ class StochasticVolatility(Model):
         self.T = int(self.r.shape[0])
         self.dim = self.T + 2
         self.noncentred = bool(noncentred)
+        self.sigma_rate, self.nu_rate = float(sigma_rate), float(nu_rate)
+        if not (self.sigma_rate > 0 and self.nu_rate > 0 and math.isfinite(self.sigma_rate)
+                and math.isfinite(self.nu_rate)):
+            raise ValueError("sigma_rate and nu_rate must be finite and positive")
         if self.noncentred:
             self.site_sizes = (("nu", 1, "exp"), ("sigma", 1, "exp"), ("zeta", self.T, None))
         else:
             self.site_sizes = (("nu", 1, "exp"), ("s", self.T, None), ("sigma", 1, "exp"))
 
     def attach(self, handle):
-        _lib.check(_lib.load().nax_model_sv(handle, _lib.ptr(self.r), self.T,
-                                            int(self.noncentred)), "sv model")
+        _lib.check(_lib.load().nax_model_sv_ex(handle, _lib.ptr(self.r), self.T,
+                                               int(self.noncentred), self.sigma_rate,
+                                               self.nu_rate), "sv model")
 
     def postprocess(self, z):
         out = super().postprocess(z)
@@ -235,6 +241,8 @@ def _device_value_and_grad(model, z):
     z [C, dim] from the library's fused potential + adjoint kernels."""
     from .infer.hmc import Engine  # local: infer imports this module
     C = int(z.shape[0])
+    if hasattr(model, "bind"):
+        model.bind(C)
     cache = model.__dict__.setdefault("_vg_engines", {})
     eng = cache.get(C)
     if eng is None:

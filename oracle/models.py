@@ -44,19 +44,19 @@ SV_SIGMA_RATE = 50.0
 SV_NU_RATE = 0.1
 
 
-def sv_potential(z, returns):
+def sv_potential(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
     z = np.asarray(z, dtype=np.float64)
     T = returns.shape[0]
     u_nu, s, u_sig = z[0], z[1:1 + T], z[1 + T]
     nu, sig = np.exp(u_nu), np.exp(u_sig)
-    lp = dist.exponential_log_prob(sig, SV_SIGMA_RATE) + u_sig
+    lp = dist.exponential_log_prob(sig, sigma_rate) + u_sig
     lp += dist.gaussian_random_walk_log_prob(s, sig)
-    lp += dist.exponential_log_prob(nu, SV_NU_RATE) + u_nu
+    lp += dist.exponential_log_prob(nu, nu_rate) + u_nu
     lp += np.sum(dist.student_t_log_prob(returns, nu, 0.0, np.exp(s)))
     return -lp
 
 
-def sv_potential_grad(z, returns):
+def sv_potential_grad(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
     z = np.asarray(z, dtype=np.float64)
     T = returns.shape[0]
     u_nu, s, u_sig = z[0], z[1:1 + T], z[1 + T]
@@ -68,16 +68,16 @@ def sv_potential_grad(z, returns):
     gs[0] -= s[0] / sig ** 2
     gs[1:] -= d / sig ** 2
     gs[:-1] += d / sig ** 2
-    g_usig = S2 / sig ** 2 - T - SV_SIGMA_RATE * sig + 1.0
+    g_usig = S2 / sig ** 2 - T - sigma_rate * sig + 1.0
     # StudentT likelihood with scale exp(s)
     y = returns * np.exp(-s)
     y2 = y * y
     gs += (nu + 1.0) * y2 / (nu + y2) - 1.0
     dnu = np.sum(-0.5 * np.log1p(y2 / nu) + 0.5 * (nu + 1.0) * y2 / (nu * (nu + y2))
                  - 0.5 / nu - 0.5 * digamma(0.5 * nu) + 0.5 * digamma(0.5 * (nu + 1.0)))
-    g_unu = nu * dnu - SV_NU_RATE * nu + 1.0
+    g_unu = nu * dnu - nu_rate * nu + 1.0
     g = np.concatenate([[g_unu], gs, [g_usig]])
-    return sv_potential(z, returns), -g
+    return sv_potential(z, returns, sigma_rate, nu_rate), -g
 
 
 # ---- non-centred hierarchical logistic regression, one group ----------------
@@ -128,20 +128,20 @@ def central_fd(f, z, eps=1e-5):
 # of examples/stochastic_volatility.py:58-66 (s_0 ~ N(0, sigma), increments
 # N(0, sigma)).
 
-def sv_nc_potential(z, returns):
+def sv_nc_potential(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
     z = np.asarray(z, dtype=np.float64)
     T = returns.shape[0]
     u_nu, u_sig, zeta = z[0], z[1], z[2:2 + T]
     nu, sig = np.exp(u_nu), np.exp(u_sig)
     s = sig * np.cumsum(zeta)
-    lp = dist.exponential_log_prob(sig, SV_SIGMA_RATE) + u_sig
+    lp = dist.exponential_log_prob(sig, sigma_rate) + u_sig
     lp += np.sum(dist.normal_log_prob(zeta))
-    lp += dist.exponential_log_prob(nu, SV_NU_RATE) + u_nu
+    lp += dist.exponential_log_prob(nu, nu_rate) + u_nu
     lp += np.sum(dist.student_t_log_prob(returns, nu, 0.0, np.exp(s)))
     return -lp
 
 
-def sv_nc_potential_grad(z, returns):
+def sv_nc_potential_grad(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
     z = np.asarray(z, dtype=np.float64)
     T = returns.shape[0]
     u_nu, u_sig, zeta = z[0], z[1], z[2:2 + T]
@@ -152,8 +152,8 @@ def sv_nc_potential_grad(z, returns):
     a = 1.0 - (nu + 1.0) * y2 / (nu + y2)           # dU/ds_t
     suffix = np.cumsum(a[::-1])[::-1]                 # sum_{tau >= t} a_tau
     g_zeta = zeta + sig * suffix
-    g_usig = sig * np.sum(a * csum) + SV_SIGMA_RATE * sig - 1.0
+    g_usig = sig * np.sum(a * csum) + sigma_rate * sig - 1.0
     dnu = np.sum(0.5 * np.log1p(y2 / nu) - 0.5 * (nu + 1.0) * y2 / (nu * (nu + y2))
                  + 0.5 / nu + 0.5 * digamma(0.5 * nu) - 0.5 * digamma(0.5 * (nu + 1.0)))
-    g_unu = nu * dnu + SV_NU_RATE * nu - 1.0
-    return sv_nc_potential(z, returns), np.concatenate([[g_unu, g_usig], g_zeta])
+    g_unu = nu * dnu + nu_rate * nu - 1.0
+    return sv_nc_potential(z, returns, sigma_rate, nu_rate), np.concatenate([[g_unu, g_usig], g_zeta])
