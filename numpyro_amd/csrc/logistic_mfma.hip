@@ -46,8 +46,16 @@ __device__ __forceinline__ float softplus_f(float x) {
 }
 __device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
+// r = sigmoid(l) - y and u = softplus(l) - y l (logistic_terms_f) with the
+// sign test shared: r = (l >= 0 ? 1 : e) / (1 + e) - y as one fma, and
+// max(l, 0) as a select on the same test.
 __device__ __forceinline__ void logit_terms(float l, float y, float& r, float& u) {
-  logistic_terms_f(l, y, r, u);
+  const bool pos = l >= 0.0f;
+  const float e = __builtin_amdgcn_exp2f(-fabsf(l) * 1.44269504088896341f);
+  const float d = 1.0f + e;
+  const float inv = __builtin_amdgcn_rcpf(d);
+  r = fmaf(pos ? 1.0f : e, inv, -y);
+  u = fmaf(__builtin_amdgcn_logf(d), 0.693147180559945309f, fmaf(-y, l, pos ? l : 0.0f));
 }
 
 
