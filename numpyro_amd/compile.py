@@ -241,7 +241,8 @@ def _cval(v):
 
 
 def _uniform_scalar(v, want=None):
-    """The single value a constant holds everywhere (None if it varies)."""
+    """The single value a constant holds everywhere; no match if it is not a
+    constant, varies, or differs from ``want``."""
     if not _is_const(v):
         raise _NoMatch
     t = _cval(v).to(torch.float64).reshape(-1)
@@ -314,6 +315,8 @@ def _match_logistic(lat, obs):
         raise _NoMatch
     _uniform_scalar(pd.loc, 0.0)
     s = _uniform_scalar(pd.scale)
+    if not s > 0:
+        raise ValueError(f"site {bname!r}: Normal scale must be positive, got {s}")
     od = osite["fn"].base
     if not isinstance(od, dist.BernoulliLogits):
         raise _NoMatch

@@ -212,3 +212,14 @@ def test_plate_shapes():
 def test_latent_without_handler_raises():
     with pytest.raises(RuntimeError, match="has no value"):
         npa.sample("x", dist.Normal(0.0, 1.0))
+
+
+def test_invalid_prior_scale_is_an_error():
+    X, y = _logistic_data(20, 2)
+
+    def model():
+        b = npa.sample("b", dist.Normal(0.0, -1.0).expand([2]))
+        npa.sample("y", dist.Bernoulli(logits=X @ b), obs=y)
+
+    with pytest.raises(ValueError, match="scale must be positive"):
+        match_model(model)
