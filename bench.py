@@ -216,7 +216,7 @@ def main():
         "warmup_s": warm_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic_bytes(args, N, D, C),
-                     "kernel": model_kernel_name(args.x_dtype, D, C), "kernel_avg_ms": k_avg_ms,
+                     "kernel": model_kernel_name(args.x_dtype, N, D, C), "kernel_avg_ms": k_avg_ms,
                      "launches": kn, "algorithmic_flops_per_launch": flops,
                      "traffic_per": "launch over all C chains (profile_kernel); a tick's "
                                     "launch covers only the chains that asked for a gradient"},
@@ -299,10 +299,14 @@ def bench_hier(args):
     print(json.dumps(line), flush=True)
 
 
-def model_kernel_name(x_dtype, D, C):
+def model_kernel_name(x_dtype, N, D, C):
+    """The kernel the library's automatic choice runs for this workload."""
     dp = (D + 31) // 32 * 32
     if x_dtype == "bf16":
         return f"logit_bf16_pipe<{dp}>"
+    if C <= 16 and D <= 64 and N * D <= (1 << 20):
+        return (f"persistent_kernel<LogisticChainModel<{32 if D <= 32 else 64}>> "
+                "(one wavefront per chain; a launch runs 50 NUTS iterations)")
     if C < 32:
         return "logit_partial_f32 (VALU tile)"
     if C > 128:

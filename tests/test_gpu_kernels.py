@@ -195,7 +195,9 @@ def _eval(eng, lib, Z):
 
 @pytest.mark.parametrize("N, D, C", [(3001, 256, 128), (1000, 55, 40), (777, 26, 33),
                                      (65, 32, 130), (4096, 200, 256), (50001, 256, 256),
-                                     (20011, 55, 64)])
+                                     (20011, 55, 64),
+                                     # one padded row tile, D = 1, one chain, ragged D
+                                     (1, 1, 1), (5, 3, 2), (31, 33, 1)])
 def test_logistic_mfma_fp32(lib, N, D, C):
     from numpyro_amd.infer.hmc import Engine
     from numpyro_amd.models import LogisticRegression
@@ -213,7 +215,9 @@ def test_logistic_mfma_fp32(lib, N, D, C):
                                      (65, 32, 130), (4096, 200, 256),
                                      # many row tiles per block: exercises the LDS ring
                                      (50001, 256, 256), (20011, 55, 64), (9001, 26, 40),
-                                     (70003, 160, 1024)])
+                                     (70003, 160, 1024),
+                                     # one padded row tile, D = 1, one chain, ragged D
+                                     (1, 1, 1), (5, 3, 2), (31, 33, 1), (33, 255, 3)])
 def test_logistic_mfma_bf16(lib, N, D, C):
     """BF16 storage defines the model with X rounded to bf16.  Against the
     float64 oracle on that X: U to 1e-5 relative (positions enter as a bf16
