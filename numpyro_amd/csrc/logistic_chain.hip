@@ -93,11 +93,11 @@ struct LogisticChainPotential : Potential {
     if (cfg.dense)
       hipLaunchKernelGGL((persistent_kernel<LogisticChainModel<DMAX>, true, 1>), dim3(cfg.C),
                          dim3(64), 0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout,
-                         gevals);
+                         gevals, 0);
     else
       hipLaunchKernelGGL((persistent_kernel<LogisticChainModel<DMAX>, false, 1>), dim3(cfg.C),
                          dim3(64), 0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout,
-                         gevals);
+                         gevals, 0);
     mark_end(s);
     return hipGetLastError();
   }

@@ -19,13 +19,17 @@
 namespace nax {
 
 // ---- per-chain vector slots (each [Dp] fp32) --------------------------------
+// The slots every leaf touches come first (V_HOT of them): a persistent
+// launch with one wavefront per workgroup keeps the leading slots in LDS.
 enum {
-  V_Z = 0, V_G,                    // HMCState.z, z_grad
-  V_ZL, V_RL, V_GL, V_ZR, V_RR, V_GR,  // tree edges (the subtree's outer edge lives here too)
+  V_ZL = 0, V_RL, V_GL, V_ZR, V_RR, V_GR,  // tree edges (the subtree's outer edge lives here too)
   V_ZP, V_GP,                      // tree proposal
   V_RSUM,                          // tree r_sum
   V_SZP, V_SGP, V_SRSUM,           // subtree proposal and r_sum
-  V_IMM, V_MSQRT,                  // inverse mass matrix diag, mass_matrix_sqrt diag
+  V_IMM,                           // inverse mass matrix diag
+  V_HOT,
+  V_Z = V_HOT, V_G,                // HMCState.z, z_grad
+  V_MSQRT,                         // mass_matrix_sqrt diag
   V_R0,                            // momentum drawn by init_kernel
   V_TMP,                           // scratch vector of the dense-mass products
   V_CK                             // r_ckpts[depth], then r_sum_ckpts[depth]
@@ -40,8 +44,15 @@ enum {
 __device__ __forceinline__ Key K(const uint32_t* k) { return Key{k[0], k[1]}; }
 __device__ __forceinline__ void setK(uint32_t* d, Key k) { d[0] = k.k0; d[1] = k.k1; }
 
-struct ChainCtx {
+// LDSV: the first nl slots may live in LDS (one-wavefront persistent
+// launches of long-vector models).  Only that context pays for the run-time
+// slot test and for generic (flat) vector pointers; the others address the
+// global vector block directly.
+template <bool LDSV>
+struct ChainCtxT {
   float* vb;       // chain's vector block [nvec][Dp]
+  float* lb;       // LDS copy of slots [0, nl) when LDSV, else unused
+  int nl;
   double* wel;     // Welford [2][Dp] (mean, m2) fp64
   int lane, D, Dp;
   // dense mass matrix ([Dp][Dp] row-major blocks, nullptr when diagonal):
@@ -49,8 +60,12 @@ struct ChainCtx {
   float* mi;
   float* ms;
   double* m2;
-  __device__ float* v(int slot) const { return vb + (size_t)slot * Dp; }
+  __device__ float* v(int slot) const {
+    if constexpr (LDSV) return (slot < nl ? lb : vb) + (size_t)slot * Dp;
+    return vb + (size_t)slot * Dp;
+  }
 };
+using ChainCtx = ChainCtxT<false>;
 
 // A translation unit whose chains have long latent vectors (thousands of
 // entries per lane group) defines NAX_FOR_D_UNROLL before including this
@@ -68,7 +83,8 @@ __device__ __forceinline__ void wave_sync_mem() { __threadfence_block(); }
 
 // (A x)[d] for a matrix stored transposed (at[k][d] = A[d][k]): lane d walks
 // column d of `at` (coalesced across lanes) against the broadcast x[k].
-__device__ __forceinline__ float dense_mv(const ChainCtx& cx, const float* at, const float* x,
+template <class CX>
+__device__ __forceinline__ float dense_mv(const CX& cx, const float* at, const float* x,
                                           int d) {
   float a0 = 0.0f, a1 = 0.0f;
   int k = 0;
@@ -80,14 +96,16 @@ __device__ __forceinline__ float dense_mv(const ChainCtx& cx, const float* at, c
   return a0 + a1;
 }
 
-__device__ __forceinline__ void vcopy(const ChainCtx& cx, int dst, int src) {
+template <class CX>
+__device__ __forceinline__ void vcopy(const CX& cx, int dst, int src) {
   float* a = cx.v(dst);
   const float* b = cx.v(src);
   FOR_D(d) a[d] = b[d];
 }
 
 // euclidean_kinetic_energy (hmc_util.py:1183-1200): 0.5 r . (M^-1 r)
-__device__ __forceinline__ double kinetic(const ChainCtx& cx, int r_slot) {
+template <class CX>
+__device__ __forceinline__ double kinetic(const CX& cx, int r_slot) {
   const float* r = cx.v(r_slot);
   double acc = 0.0;
   if (cx.mi) {
@@ -102,7 +120,8 @@ __device__ __forceinline__ double kinetic(const ChainCtx& cx, int r_slot) {
 
 // _is_turning (hmc_util.py:710-746) with r_sum given as rs - rs_ck + r_ck when
 // rs_ck != nullptr (the iterative subtree form, hmc_util.py:961-981).
-__device__ __forceinline__ bool is_turning(const ChainCtx& cx, const float* rl, const float* rr,
+template <class CX>
+__device__ __forceinline__ bool is_turning(const CX& cx, const float* rl, const float* rr,
                            const float* rs, const float* rs_ck, const float* r_ck) {
   const float* m = cx.v(V_IMM);
   double a = 0.0, b = 0.0;
@@ -127,7 +146,8 @@ __device__ __forceinline__ bool is_turning(const ChainCtx& cx, const float* rl, 
 // s = sqrt(M) (the slot V_MSQRT); find_reasonable_step_size passes the inverse
 // mass matrix in that argument (hmc_util.py:359), hence the slot parameter.
 // Dense mass: r = mass_matrix_sqrt . eps (hmc.py:107-109).
-__device__ __forceinline__ void draw_momentum(const ChainCtx& cx, int dst, Key k, bool part,
+template <class CX>
+__device__ __forceinline__ void draw_momentum(const CX& cx, int dst, Key k, bool part,
                               int scale_slot = V_MSQRT) {
   float* r = cx.v(dst);
   if (cx.mi) {
@@ -145,7 +165,8 @@ __device__ __forceinline__ void draw_momentum(const ChainCtx& cx, int dst, Key k
 
 // First half of velocity_verlet (hmc_util.py:289-297) on an edge, in place:
 // r <- r - eps/2 g; z <- z + eps M^-1 r; z is also written to the request.
-__device__ __forceinline__ void leapfrog_half(const ChainCtx& cx, int zs, int rs, int gs, float eps,
+template <class CX>
+__device__ __forceinline__ void leapfrog_half(const CX& cx, int zs, int rs, int gs, float eps,
                               float* zreq) {
   float* z = cx.v(zs);
   float* r = cx.v(rs);
@@ -172,7 +193,8 @@ __device__ __forceinline__ void leapfrog_half(const ChainCtx& cx, int zs, int rs
 }
 
 // Second half (hmc_util.py:298-309): g <- grad(z); r <- r - eps/2 g.
-__device__ __forceinline__ void leapfrog_finish(const ChainCtx& cx, int rs, int gs, float eps,
+template <class CX>
+__device__ __forceinline__ void leapfrog_finish(const CX& cx, int rs, int gs, float eps,
                                 const float* gin) {
   float* r = cx.v(rs);
   float* g = cx.v(gs);
@@ -190,7 +212,8 @@ __device__ void da_init(ChainState& S, double prox) {
   S.da_x = 0.0; S.da_xavg = 0.0; S.da_gavg = 0.0; S.da_t = 0; S.da_prox = prox;
 }
 
-__device__ void welford_reset(const ChainCtx& cx, ChainState& S) {
+template <class CX>
+__device__ void welford_reset(const CX& cx, ChainState& S) {
   double* mean = cx.wel;
   double* m2 = cx.wel + cx.Dp;
   FOR_D(d) { mean[d] = 0.0; m2[d] = 0.0; }
@@ -200,7 +223,8 @@ __device__ void welford_reset(const ChainCtx& cx, ChainState& S) {
 // ---- dense mass matrix (dense_mass=True) --------------------------------------
 // welford_covariance update, dense (hmc_util.py:171-196): with
 // dpre = x - mean, mean += dpre / n, dpost = x - mean: m2 += outer(dpost, dpre).
-__device__ void welford_dense_update(const ChainCtx& cx, const float* z, int n) {
+template <class CX>
+__device__ void welford_dense_update(const CX& cx, const float* z, int n) {
   double* mean = cx.wel;
   double* dpost = cx.wel + cx.Dp;
   double* dpre = cx.wel + 2 * cx.Dp;
@@ -228,7 +252,8 @@ __device__ void welford_dense_update(const ChainCtx& cx, const float* z, int n) 
 // triangular.  U overwrites A's upper triangle, S's strict lower triangle its
 // lower one (S's diagonal is 1 / diag U); S is then written to cx.ms
 // transposed (fp32).  One wavefront, O(D^3 / 3) fp64 operations.
-__device__ void dense_factor(const ChainCtx& cx) {
+template <class CX>
+__device__ void dense_factor(const CX& cx) {
   double* A = cx.m2;
   const int D = cx.D;
   const size_t ld = cx.Dp;
@@ -271,7 +296,8 @@ __device__ void dense_factor(const ChainCtx& cx) {
   wave_sync_mem();
 }
 
-__device__ void dense_zero(const ChainCtx& cx) {
+template <class CX>
+__device__ void dense_zero(const CX& cx) {
   for (int i = 0; i < cx.D; ++i) FOR_D(j) cx.m2[(size_t)i * cx.Dp + j] = 0.0;
   FOR_D(d) cx.wel[d] = 0.0;
   wave_sync_mem();
@@ -279,7 +305,8 @@ __device__ void dense_zero(const ChainCtx& cx) {
 
 // Window end, dense: cov = m2 / (n - 1), Stan regularisation
 // (hmc_util.py:215-226), M^-1 = cov, factors of cov, Welford reset.
-__device__ void dense_window_end(const ChainCtx& cx, double n, bool regularize) {
+template <class CX>
+__device__ void dense_window_end(const CX& cx, double n, bool regularize) {
   const size_t ld = cx.Dp;
   for (int i = 0; i < cx.D; ++i)
     FOR_D(j) {
@@ -297,7 +324,8 @@ __device__ void dense_window_end(const ChainCtx& cx, double n, bool regularize) 
 // window that ends at t is a middle window and find_reasonable_step_size must
 // run now (its key is left in S.fs_key); the dual-averaging restart of
 // _update_at_window_end (hmc_util.py:618-623) then happens when it returns.
-__device__ __forceinline__ bool adapt_update(const EngineCfg& cfg, const ChainCtx& cx, ChainState& S, int t,
+template <class CX>
+__device__ __forceinline__ bool adapt_update(const EngineCfg& cfg, const CX& cx, ChainState& S, int t,
                              double accept_prob) {
   const bool part = cfg.partitionable != 0;
   Key ak = K(S.adapt_key);
@@ -374,7 +402,8 @@ __device__ __forceinline__ void fs_begin(ChainState& S, int ret) {
 }
 
 // ---- collection (fori_collect, numpyro/util.py:384-403) ---------------------
-__device__ void collect(const EngineCfg& cfg, const Collect& col, const ChainCtx& cx,
+template <class CX>
+__device__ void collect(const EngineCfg& cfg, const Collect& col, const CX& cx,
                         const ChainState& S, int c, int64_t it) {
   if (col.size <= 0) return;
   int64_t rel = it - col.start_idx;
@@ -406,9 +435,10 @@ __device__ __forceinline__ double clip_exp_neg(double delta) {
 }
 
 // ---- the state machine -------------------------------------------------------
+template <class CX>
 __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collect& col,
                                               int64_t target, ChainState& S,
-                                              const ChainCtx& cx, int c, float* zr,
+                                              const CX& cx, int c, float* zr,
                                               const float* gi, double pe_in, int& evals) {
   const bool part = cfg.partitionable != 0;
   const int CKR = V_CK, CKS = V_CK + cfg.max_depth_alloc;
@@ -713,14 +743,16 @@ __device__ __forceinline__ void copy_state(ChainState& dst, const ChainState& sr
 
 // DENSE = false gives the diagonal-mass kernels a context whose dense
 // pointers are compile-time null, so the dense branches fold away.
-template <bool DENSE = true>
-__device__ __forceinline__ ChainCtx make_ctx(const EngineCfg& cfg, int c, float* vec,
-                                             double* welf) {
-  ChainCtx cx;
+template <bool DENSE = true, bool LDSV = false>
+__device__ __forceinline__ ChainCtxT<LDSV> make_ctx(const EngineCfg& cfg, int c, float* vec,
+                                                    double* welf) {
+  ChainCtxT<LDSV> cx;
   cx.lane = threadIdx.x & (NAX_WAVE - 1);
   cx.D = cfg.chain_dim ? cfg.chain_dim[c] : cfg.D;
   cx.Dp = cfg.Dp;
   cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
+  cx.lb = nullptr;
+  cx.nl = 0;
   cx.wel = welf + (size_t)c * 3 * cfg.Dp;
   const size_t blk = (size_t)c * cfg.Dp * cfg.Dp;
   cx.mi = (DENSE && cfg.dense) ? cfg.dimm_t + blk : nullptr;
@@ -747,21 +779,43 @@ perchain_eval_kernel(M m, const float* __restrict__ z, int ldz, int C, double* _
   if (lane == 0) pe[c] = u;
 }
 
+// Vector slots a persistent launch with one wavefront per workgroup keeps in
+// LDS: the leading slots (the per-leaf ones first) that fit in the CU's
+// 160 KB next to the kernel's static LDS, all of them when the chain's
+// vectors are short.  The host passes nl * Dp * 4 bytes of dynamic LDS.
+inline int persistent_lds_slots(int Dp, int nvec) {
+  const int budget = 160 * 1024 - 2048;
+  const int fit = budget / (Dp * 4);
+  return fit >= nvec ? nvec : (fit >= V_HOT ? V_HOT : fit);
+}
+
 // The whole sampling loop of one chain in one launch: the wave alternates
 // advance_chain() and its own potential until the chain reaches `target`.
 // WPB wavefronts (chains) per workgroup: one, for latency-bound chains with
 // long vectors, spreads the chains over as many CUs (and their L1 caches and
-// address units) as there are chains.
-template <class M, bool DENSE, int WPB>
+// address units) as there are chains.  LDSV (with WPB = 1): the first nl
+// vector slots live in LDS for the launch (copied in and back out), so a
+// leaf's vector passes run at LDS rather than L2 latency; for models with
+// long latent vectors.
+template <class M, bool DENSE, int WPB, bool LDSV = false>
 __global__ void __launch_bounds__(64 * WPB)
 persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
                   ChainState* __restrict__ states, float* __restrict__ vec,
                   double* __restrict__ welf, float* __restrict__ zreq, float* __restrict__ gout,
-                  double* __restrict__ peout, int64_t* __restrict__ grad_evals) {
+                  double* __restrict__ peout, int64_t* __restrict__ grad_evals, int nl) {
   __shared__ ChainState sS[WPB];
+  extern __shared__ float nax_slots_lds[];
   const int c = wave_chain(WPB);
   if (c >= cfg.C) return;
-  const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
+  ChainCtxT<LDSV> cx = make_ctx<DENSE, LDSV>(cfg, c, vec, welf);
+  const int nlds = (LDSV && WPB == 1) ? nl : 0;
+  if (nlds > 0) {
+    const int n = nlds * cfg.Dp;
+    for (int i = cx.lane; i < n; i += NAX_WAVE) nax_slots_lds[i] = cx.vb[i];
+    __syncthreads();
+    cx.lb = nax_slots_lds;
+    cx.nl = nlds;
+  }
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
   ChainState& S = sS[threadIdx.x / NAX_WAVE];
@@ -772,6 +826,11 @@ persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
     __threadfence_block();
     pe = m.value_and_grad(zr, gi, cx.lane);
     __threadfence_block();
+  }
+  if (nlds > 0) {
+    __syncthreads();
+    const int n = nlds * cfg.Dp;
+    for (int i = cx.lane; i < n; i += NAX_WAVE) cx.vb[i] = nax_slots_lds[i];
   }
   copy_state(states[c], S, cx.lane);
   if (cx.lane == 0) {

@@ -191,14 +191,25 @@ struct SVPotential : Potential {
   hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
                             int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
                             float* gout, double* peout, int64_t* gevals) override {
+    // one chain per workgroup: each chain gets a CU (and its LDS) to itself
+    // while C <= #CUs
+    const int nl = persistent_lds_slots(cfg.Dp, cfg.nvec);
+    const size_t lds = (size_t)nl * cfg.Dp * sizeof(float);
+    hipError_t e = cfg.dense
+        ? hipFuncSetAttribute((const void*)persistent_kernel<SVModel, true, 1, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+        : hipFuncSetAttribute((const void*)persistent_kernel<SVModel, false, 1, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
     mark_begin(s);
-    // one chain per workgroup: each chain gets a CU to itself while C <= #CUs
     if (cfg.dense)
-      hipLaunchKernelGGL((persistent_kernel<SVModel, true, 1>), dim3(cfg.C), dim3(64), 0, s, m,
-                         cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+      hipLaunchKernelGGL((persistent_kernel<SVModel, true, 1, true>), dim3(cfg.C), dim3(64),
+                         lds, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals,
+                         nl);
     else
-      hipLaunchKernelGGL((persistent_kernel<SVModel, false, 1>), dim3(cfg.C), dim3(64), 0, s, m,
-                         cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);
+      hipLaunchKernelGGL((persistent_kernel<SVModel, false, 1, true>), dim3(cfg.C), dim3(64),
+                         lds, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals,
+                         nl);
     mark_end(s);
     return hipGetLastError();
   }
