@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4"])
     ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--bf16-kernel", default="two-copy", choices=["two-copy", "single"],
+                    help="bf16 storage: two X copies (fp16 residuals) or one (bf16 residuals)")
     ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--fp32-steps", type=int, default=24,
@@ -137,7 +139,9 @@ def main():
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
     z0 = torch.zeros((C, D), dtype=torch.float32, device="cuda")  # init_to_value(0)
 
-    model = LogisticRegression(X, y, x_dtype=args.x_dtype)
+    model = LogisticRegression(
+        X, y, x_dtype=args.x_dtype,
+        kernel="single" if (args.x_dtype == "bf16" and args.bf16_kernel == "single") else "auto")
     eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
@@ -216,7 +220,7 @@ def main():
         "warmup_s": warm_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic_bytes(args, N, D, C),
-                     "kernel": model_kernel_name(args.x_dtype, N, D, C), "kernel_avg_ms": k_avg_ms,
+                     "kernel": model_kernel_name(args.x_dtype, N, D, C, args.bf16_kernel == "single"), "kernel_avg_ms": k_avg_ms,
                      "launches": kn, "algorithmic_flops_per_launch": flops,
                      "traffic_per": "launch over all C chains (profile_kernel); a tick's "
                                     "launch covers only the chains that asked for a gradient"},
@@ -299,11 +303,11 @@ def bench_hier(args):
     print(json.dumps(line), flush=True)
 
 
-def model_kernel_name(x_dtype, N, D, C):
+def model_kernel_name(x_dtype, N, D, C, single=False):
     """The kernel the library's automatic choice runs for this workload."""
     dp = (D + 31) // 32 * 32
     if x_dtype == "bf16":
-        return f"logit_bf16_pipe<{dp}>"
+        return f"logit_bf16_pipe<{dp}, {'true' if single else 'false'}>"
     if C <= 16 and D <= 64 and N * D <= (1 << 20):
         return (f"persistent_kernel<LogisticChainModel<{32 if D <= 32 else 64}>> "
                 "(one wavefront per chain; a launch runs 50 NUTS iterations)")
@@ -321,6 +325,8 @@ _MEASURED_TRAFFIC = {("bf16", 2_000_000, 256, 1024): 2_016_666 * 1024 + 33_024 *
 
 
 def traffic_bytes(args, N, D, C):
+    if args.x_dtype == "bf16" and args.bf16_kernel == "single":
+        return None  # not measured for the single-copy form
     return _MEASURED_TRAFFIC.get((args.x_dtype, N, D, C))
 
 

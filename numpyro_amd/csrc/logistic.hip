@@ -248,14 +248,22 @@ struct LogisticPotential : Potential {
 };
 
 Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
-                              double prior_scale, bool bf16, hipStream_t s, std::string* err);
+                              double prior_scale, bool bf16, bool single, hipStream_t s,
+                              std::string* err);
 
 // fuse_variant: 0 = auto (matrix cores when num_chains >= 32), 1 = VALU tile
-// kernel, 2 = matrix cores.  bf16 always runs on the matrix cores.
+// kernel, 2 = matrix cores, 3 = one wavefront per chain.  bf16 always runs on
+// the matrix cores; with fuse_variant 4 it keeps one X copy (bf16
+// residuals in X^T R).
 Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
                          int x_dtype, int fuse_variant, int num_chains, hipStream_t s,
                          std::string* err) {
-  if (x_dtype == NAX_X_BF16) return make_logistic_mfma(x, y, n, d, prior_scale, true, s, err);
+  if (x_dtype == NAX_X_BF16)
+    return make_logistic_mfma(x, y, n, d, prior_scale, true, fuse_variant == 4, s, err);
+  if (fuse_variant == 4) {
+    *err = "kernel 4 (single-copy bf16) needs bf16 storage";
+    return nullptr;
+  }
   if (x_dtype != NAX_X_F32) {
     *err = "x_dtype not supported";
     return nullptr;
@@ -267,7 +275,7 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
     return make_logistic_chain(x, y, n, d, prior_scale, err);
   const bool mfma = fuse_variant == 2 || (fuse_variant == 0 && num_chains >= 32 && d <= 256);
   if (mfma && !(fuse_variant == 0 && num_chains <= 128 && d <= 320))
-    return make_logistic_mfma(x, y, n, d, prior_scale, false, s, err);
+    return make_logistic_mfma(x, y, n, d, prior_scale, false, false, s, err);
   if (d > 320) {
     *err = "the fp32 logistic tile kernel supports dim <= 320";
     return nullptr;
@@ -318,7 +326,7 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
     p->rb_pick = best_rb;
     if (mfma) {
       std::string e2;
-      Potential* q = make_logistic_mfma(x, y, n, d, prior_scale, false, s, &e2);
+      Potential* q = make_logistic_mfma(x, y, n, d, prior_scale, false, false, s, &e2);
       if (q) {
         if (time_it(q) < best_ms) best = q;
         else delete q;

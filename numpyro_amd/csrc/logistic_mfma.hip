@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "nax_device.h"
 #include "nax_internal.h"
@@ -350,7 +351,7 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // and then waits only for the loads issued two iterations earlier -- every
 // load has three iterations (~4 us) to arrive.  GEMM2 runs in fp16: X (bf16
 // values) is exact in fp16 and R gets 11 mantissa bits in a single MFMA.
-template <int DP>
+template <int DP, bool SINGLE>
 __global__ void __launch_bounds__(256, 1)
 logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
                 const float* __restrict__ ypad, int64_t N, int64_t ntile,
@@ -367,11 +368,15 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   constexpr int PP = (KK + 3) / 4;
   static_assert(ABYTES == KK * 64 * 16 && BBYTES == KK * 64 * 16, "part sizes");
   // loads per iteration of the wave that issues the fewest (wave 3)
-  constexpr int NMIN = 2 * (KK / 4) + 1;
+  constexpr int NMIN = (SINGLE ? 1 : 2) * (KK / 4) + 1;
   (void)TBYTES;
-  __shared__ __attribute__((aligned(16))) char lds_a[4][ABYTES];
-  __shared__ __attribute__((aligned(16))) char lds_b[4][BBYTES];
+  // SINGLE: GEMM2 reads X transposed from the A1 tile, which then lives
+  // from iteration t-1 (GEMM1) to t+1 (GEMM2): 8 A1 slots, no B2 ring.
+  constexpr int NA = SINGLE ? 8 : 4;
+  __shared__ __attribute__((aligned(16))) char lds_a[NA][ABYTES];
+  __shared__ __attribute__((aligned(16))) char lds_b[SINGLE ? 1 : 4][SINGLE ? 16 : BBYTES];
   __shared__ __attribute__((aligned(16))) char lds_y[4][128];
+  typedef typename std::conditional<SINGLE, bf16x8, f16x8>::type rfrag;
 
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -429,19 +434,21 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   auto tile_of = [&](int64_t i) { return rb + (i < n ? i : n - 1) * (int64_t)RB; };
   auto issue_a = [&](int64_t i) {
     const uint4* ga = (const uint4*)(A1 + (size_t)tile_of(i) * KK * 64);
-    char* base = lds_a[i & 3];
+    char* base = lds_a[i & (NA - 1)];
 #pragma unroll
     for (int p = 0; p < PP; ++p)
       if (KK % 4 == 0 || 4 * p + wave < KK)
         lds_dma16(ga + tid + 256 * p, lds_addr(base + 16 * (256 * p + 64 * wave)));
   };
   auto issue_b = [&](int64_t i) {
-    const uint4* gb = (const uint4*)(B2 + (size_t)tile_of(i) * 2 * DT * 64);
-    char* base = lds_b[i & 3];
+    if constexpr (!SINGLE) {
+      const uint4* gb = (const uint4*)(B2 + (size_t)tile_of(i) * 2 * DT * 64);
+      char* base = lds_b[i & 3];
 #pragma unroll
-    for (int p = 0; p < PP; ++p)
-      if (KK % 4 == 0 || 4 * p + wave < KK)
-        lds_dma16(gb + tid + 256 * p, lds_addr(base + 16 * (256 * p + 64 * wave)));
+      for (int p = 0; p < PP; ++p)
+        if (KK % 4 == 0 || 4 * p + wave < KK)
+          lds_dma16(gb + tid + 256 * p, lds_addr(base + 16 * (256 * p + 64 * wave)));
+    }
   };
   auto issue_y = [&](int64_t i) {
     if (l < 2) lds_dma16(ypad + 32 * tile_of(i) + 4 * (2 * wave + l), lds_addr(lds_y[i & 3] + 32 * wave));
@@ -463,8 +470,39 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
   };
+  // GEMM2's B operand for MFMA (m2, dt) of tile i: X rows 16 m2 + 8 (j / 4)
+  // + 4 h + j % 4 (the K order that makes R^T the GEMM1 accumulator
+  // registers), column 32 dt + lane % 32.  Two-copy form: one 16-byte read
+  // of the fragment-ordered B2 copy.  Single-copy form: two
+  // ds_read_b64_tr_b16 of the A1 tile, whose rows hold 8 consecutive
+  // columns.  In each 16-lane group, lane q of the transposed read returns
+  // element q % 4 of the 8-byte chunks addressed by lanes q / 4 + 4 k
+  // (k = 0..3), so lane q addresses row (q / 4) of its group's 4-row block at
+  // columns 4 (q % 4) .. 4 (q % 4) + 3 of its group's 16-column half.
+  const int jl = l & 15, gl = l >> 4;
+  const int tr_off =
+      (((gl & 1) * 64 + ((jl & 3) >> 1) * 32 + 4 * (gl >> 1) + (jl >> 2)) * 16) + 8 * (jl & 1);
+  auto bfrag = [&](int64_t i, int m2, int dt) -> rfrag {
+    if constexpr (SINGLE) {
+      typedef short v4s __attribute__((ext_vector_type(4)));
+      typedef short v8s __attribute__((ext_vector_type(8)));
+      const char* base = lds_a[i & (NA - 1)] + tr_off + dt * 2048 + m2 * 256;
+      const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+          (__attribute__((address_space(3))) v4s*)(base));
+      const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+          (__attribute__((address_space(3))) v4s*)(base + 128));
+      const v8s w = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+      return __builtin_bit_cast(rfrag, w);
+    } else {
+      return ((const f16x8*)lds_b[i & 3])[(m2 * DT + dt) * 64 + l];
+    }
+  };
+  auto mma2 = [&](rfrag a, rfrag b, const f32x16& acc) -> f32x16 {
+    if constexpr (SINGLE) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+  };
   auto gemm1 = [&](int64_t i) {
-    const bf16x8* la = (const bf16x8*)lds_a[i & 3];
+    const bf16x8* la = (const bf16x8*)lds_a[i & (NA - 1)];
     f32x16 L;
 #pragma unroll
     for (int v = 0; v < 16; ++v) L[v] = 0.0f;
@@ -476,12 +514,11 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     }
     return L;
   };
-  auto gemm2 = [&](int64_t i, f16x8 r0, f16x8 r1) {
-    const f16x8* lb = (const f16x8*)lds_b[i & 3];
+  auto gemm2 = [&](int64_t i, rfrag r0, rfrag r1) {
 #pragma unroll
     for (int dt = 0; dt < DT; ++dt) {
-      g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(r0, lb[dt * 64 + l], g[dt], 0, 0, 0);
-      g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(r1, lb[(DT + dt) * 64 + l], g[dt], 0, 0, 0);
+      g[dt] = mma2(r0, bfrag(i, 0, dt), g[dt]);
+      g[dt] = mma2(r1, bfrag(i, 1, dt), g[dt]);
     }
   };
   // Rows past N have X = 0 and y = 1/2, so r = 0 exactly and u = fp32(ln 2)
@@ -499,7 +536,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       yv[4 * q + 3] = yq.w;
     }
   };
-  auto epilogue = [&](int64_t i, const f32x16& L, f16x8& r0, f16x8& r1) {
+  auto epilogue = [&](int64_t i, const f32x16& L, rfrag& r0, rfrag& r1) {
     float yv[16];
     load_y(i, yv);
     float us = 0.0f;
@@ -508,17 +545,16 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       float r, uu;
       logit_terms(L[v], yv[v], r, uu);
       us += uu;
-      if (v < 8) r0[v] = (_Float16)r;
-      else r1[v - 8] = (_Float16)r;
+      if (v < 8) r0[v] = (typename std::remove_reference<decltype(r0[0])>::type)r;
+      else r1[v - 8] = (typename std::remove_reference<decltype(r1[0])>::type)r;
     }
     u += (double)us;
   };
   // steady-state step: epilogue(i) || GEMM1(i+1) || GEMM2(i-1), interleaved
   // element by element in source order.
-  auto fused_step = [&](int64_t i, const f32x16& Lc, f16x8 rp0, f16x8 rp1, f16x8& r0,
-                        f16x8& r1) {
-    const bf16x8* la = (const bf16x8*)lds_a[(i + 1) & 3];
-    const f16x8* lb = (const f16x8*)lds_b[(i - 1) & 3];
+  auto fused_step = [&](int64_t i, const f32x16& Lc, rfrag rp0, rfrag rp1, rfrag& r0,
+                        rfrag& r1) {
+    const bf16x8* la = (const bf16x8*)lds_a[(i + 1) & (NA - 1)];
     float yv[16];
     load_y(i, yv);
     f32x16 Ln;
@@ -536,22 +572,20 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
 #pragma unroll
       for (int j = e * 2 * DT / 16; j < (e + 1) * 2 * DT / 16; ++j) {
         const int m2 = j / DT, dt = j % DT;
-        g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(m2 ? rp1 : rp0, lb[(m2 * DT + dt) * 64 + l],
-                                                       g[dt], 0, 0, 0);
+        g[dt] = mma2(m2 ? rp1 : rp0, bfrag(i - 1, m2, dt), g[dt]);
       }
       float r, uu;
       logit_terms(Lc[e], yv[e], r, uu);
       us += uu;
-      if (e < 8) r0[e] = (_Float16)r;
-      else r1[e - 8] = (_Float16)r;
+      if (e < 8) r0[e] = (typename std::remove_reference<decltype(r0[0])>::type)r;
+      else r1[e - 8] = (typename std::remove_reference<decltype(r1[0])>::type)r;
     }
     u += (double)us;
     return Ln;
   };
-  auto fused_step256 = [&](int64_t i, const f32x16& Lc, f16x8 rp0, f16x8 rp1, f16x8& r0,
-                        f16x8& r1) {
-    const bf16x8* la = (const bf16x8*)lds_a[(i + 1) & 3];
-    const f16x8* lb = (const f16x8*)lds_b[(i - 1) & 3];
+  auto fused_step256 = [&](int64_t i, const f32x16& Lc, rfrag rp0, rfrag rp1, rfrag& r0,
+                        rfrag& r1) {
+    const bf16x8* la = (const bf16x8*)lds_a[(i + 1) & (NA - 1)];
     float yv[16];
     load_y(i, yv);
     f32x16 Ln;
@@ -563,28 +597,28 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     // chunk ahead.  Scheduling barriers keep each chunk's VALU work next to
     // its MFMAs so that it runs in their shadow.
     bf16x8 af = la[l];
-    f16x8 bf = lb[l];
+    rfrag bf = bfrag(i - 1, 0, 0);
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       bf16x8 af_n;
-      f16x8 bf_n;
+      rfrag bf_n;
       if (e + 1 < 16) {
         af_n = la[(e + 1) * 64 + l];
         const int jn = e + 1, m2n = jn / DT, dtn = jn % DT;
-        bf_n = lb[(m2n * DT + dtn) * 64 + l];
+        bf_n = bfrag(i - 1, m2n, dtn);
       }
       Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zh[e], Ln, 0, 0, 0);
       {
         const int m2 = e / DT, dt = e % DT;
-        g[dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(m2 ? rp1 : rp0, bf, g[dt], 0, 0, 0);
+        g[dt] = mma2(m2 ? rp1 : rp0, bf, g[dt]);
       }
       Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zlo[e], Ln, 0, 0, 0);
       float r, uu;
       logit_terms(Lc[e], yv[e], r, uu);
       us += uu;
       asm volatile("" : "+v"(us));  // keeps this element's loss term in the chunk
-      if (e < 8) r0[e] = (_Float16)r;
-      else r1[e - 8] = (_Float16)r;
+      if (e < 8) r0[e] = (typename std::remove_reference<decltype(r0[0])>::type)r;
+      else r1[e - 8] = (typename std::remove_reference<decltype(r1[0])>::type)r;
       if (e + 1 < 16) {
         af = af_n;
         bf = bf_n;
@@ -609,11 +643,11 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     issue_y(2);
     wait_all();
     f32x16 Lc = gemm1(0);
-    f16x8 rp0, rp1;
+    rfrag rp0, rp1;
     {  // i = 0: epilogue(0) + gemm1(1)
       __syncthreads();  // A1 slot 0 is refilled next: every wave has read tile 0
       issue_group(0);
-      f16x8 r0, r1;
+      rfrag r0, r1;
       epilogue(0, Lc, r0, r1);
       if (n > 1) Lc = gemm1(1);
       rp0 = r0;
@@ -622,7 +656,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     }
     for (int64_t i = 1; i + 1 < n; ++i) {  // steady state, branch-free body
       issue_group(i);
-      f16x8 r0, r1;
+      rfrag r0, r1;
       if constexpr (DP == 256) Lc = fused_step256(i, Lc, rp0, rp1, r0, r1);
       else Lc = fused_step(i, Lc, rp0, rp1, r0, r1);
       rp0 = r0;
@@ -630,7 +664,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       wait_group();
     }
     if (n > 1) {  // i = n-1: epilogue(n-1) + gemm2(n-2)
-      f16x8 r0, r1;
+      rfrag r0, r1;
       epilogue(n - 1, Lc, r0, r1);
       gemm2(n - 2, rp0, rp1);
       rp0 = r0;
@@ -674,6 +708,7 @@ struct LogisticMfma : Potential {
   int d = 0, dp = 0;
   double prior_scale = 1.0;
   bool bf16 = true;
+  bool single = false;  // bf16 only: one X copy, GEMM2 transposed from it, bf16 R
   void* a1 = nullptr;
   void* b2 = nullptr;
   float* ypad = nullptr;
@@ -686,7 +721,9 @@ struct LogisticMfma : Potential {
     hipFree(a1); hipFree(b2); hipFree(ypad); hipFree(wsg); hipFree(wsu);
   }
   int dim() const override { return d; }
-  const char* name() const override { return bf16 ? "logit_mfma_bf16" : "logit_mfma_f32"; }
+  const char* name() const override {
+    return bf16 ? (single ? "logit_mfma_bf16_single" : "logit_mfma_bf16") : "logit_mfma_f32";
+  }
   double alg_flops(int C) const override { return 4.0 * (double)n * d * C; }
   double alg_bytes(int C) const override { return (bf16 ? 2.0 : 4.0) * (double)n * d + 4.0 * n; }
 
@@ -697,8 +734,10 @@ struct LogisticMfma : Potential {
     const size_t per = (size_t)ntile * 32 * dp * elt;
     hipError_t e = hipMalloc(&a1, per);
     if (e != hipSuccess) return e;
-    e = hipMalloc(&b2, per);
-    if (e != hipSuccess) return e;
+    if (!single) {
+      e = hipMalloc(&b2, per);
+      if (e != hipSuccess) return e;
+    }
     e = hipMalloc(&ypad, sizeof(float) * ntile * 32);
     if (e != hipSuccess) return e;
     int dev = 0;
@@ -706,7 +745,9 @@ struct LogisticMfma : Potential {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) num_cu = prop.multiProcessorCount;
     const int blocks = 4096;
-    if (bf16) {
+    if (bf16 && single) {
+      hipLaunchKernelGGL((prep_a1<__bf16, 16>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)a1);
+    } else if (bf16) {
       unsigned* dmax = nullptr;
       e = hipMallocAsync((void**)&dmax, sizeof(unsigned), s);
       if (e != hipSuccess) return e;
@@ -776,22 +817,30 @@ struct LogisticMfma : Potential {
     case 256: NAX_L(FORM, 256); break;  \
     default: return hipErrorInvalidValue; \
   }
-#define NAX_P(DPV)                                                                             \
-  hipLaunchKernelGGL((logit_bf16_pipe<DPV>), dim3(nblk), dim3(256), 0, s, (const bf16x8*)a1,   \
-                     (const f16x8*)b2, ypad, n, ntile, z, ldz, d, C, wsg, wsu, act, nact)
-    if (bf16) {
-      switch (dp) {
-        case 32: NAX_P(32); break;
-        case 64: NAX_P(64); break;
-        case 96: NAX_P(96); break;
-        case 128: NAX_P(128); break;
-        case 160: NAX_P(160); break;
-        case 192: NAX_P(192); break;
-        case 224: NAX_P(224); break;
-        case 256: NAX_P(256); break;
-        default: return hipErrorInvalidValue;
-      }
-    } else { NAX_DP(FormF32) }
+#define NAX_P(DPV, SGL)                                                                        \
+  hipLaunchKernelGGL((logit_bf16_pipe<DPV, SGL>), dim3(nblk), dim3(256), 0, s,                 \
+                     (const bf16x8*)a1, (const f16x8*)b2, ypad, n, ntile, z, ldz, d, C, wsg,   \
+                     wsu, act, nact)
+#define NAX_PDP(SGL)                        \
+  switch (dp) {                             \
+    case 32: NAX_P(32, SGL); break;         \
+    case 64: NAX_P(64, SGL); break;         \
+    case 96: NAX_P(96, SGL); break;         \
+    case 128: NAX_P(128, SGL); break;       \
+    case 160: NAX_P(160, SGL); break;       \
+    case 192: NAX_P(192, SGL); break;       \
+    case 224: NAX_P(224, SGL); break;       \
+    case 256: NAX_P(256, SGL); break;       \
+    default: return hipErrorInvalidValue;   \
+  }
+    if (bf16 && single) {
+      NAX_PDP(true)
+    } else if (bf16) {
+      NAX_PDP(false)
+    } else {
+      NAX_DP(FormF32)
+    }
+#undef NAX_PDP
 #undef NAX_P
 #undef NAX_DP
 #undef NAX_L
@@ -809,7 +858,8 @@ struct LogisticMfma : Potential {
 };
 
 Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
-                              double prior_scale, bool bf16, hipStream_t s, std::string* err) {
+                              double prior_scale, bool bf16, bool single, hipStream_t s,
+                              std::string* err) {
   if (d > 256) {
     *err = "the MFMA logistic kernels support dim <= 256";
     return nullptr;
@@ -819,10 +869,12 @@ Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
   p->d = d;
   p->prior_scale = prior_scale;
   p->bf16 = bf16;
+  p->single = bf16 && single;
   hipError_t e = p->setup(x, y, s);
   if (e != hipSuccess) {
     *err = std::string("logistic MFMA setup: ") + hipGetErrorString(e) +
-           (bf16 && e == hipErrorInvalidValue ? " (the bf16 form needs max|X| < 60000)" : "");
+           (bf16 && !single && e == hipErrorInvalidValue
+                ? " (the two-copy bf16 form needs max|X| < 60000)" : "");
     delete p;
     return nullptr;
   }

@@ -383,7 +383,11 @@ extern "C" int nax_model_logistic_ex(nax_sampler* h, const float* x, const float
   if (n <= 0) { set_error("logistic model needs n > 0 rows"); return NAX_EINVAL; }
   if (!(prior_scale > 0)) { set_error("prior_scale must be positive"); return NAX_EINVAL; }
   std::string err;
-  if (kernel < 0 || kernel > 3) { set_error("kernel must be 0 (auto), 1 (tile), 2 (matrix cores) or 3 (one wavefront per chain)"); return NAX_EINVAL; }
+  if (kernel < 0 || kernel > 4) {
+    set_error("kernel must be 0 (auto), 1 (tile), 2 (matrix cores), 3 (one wavefront per chain) "
+              "or 4 (single-copy bf16)");
+    return NAX_EINVAL;
+  }
   Potential* p = make_logistic(x, y, n, d, prior_scale, x_dtype, kernel, h->cfg.C, h->stream, &err);
   if (!p) { set_error(err); return NAX_EINVAL; }
   delete h->pot;

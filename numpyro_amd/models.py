@@ -76,14 +76,16 @@ class LogisticRegression(Model):
         if x_dtype not in _lib.X_DTYPES:
             raise ValueError(f"unknown x_dtype {x_dtype!r}")
         self.x_dtype = x_dtype
-        if kernel not in ("auto", "tile", "mfma", "chain"):
-            raise ValueError("kernel must be 'auto', 'tile', 'mfma' or 'chain'")
+        if kernel not in ("auto", "tile", "mfma", "chain", "single"):
+            raise ValueError("kernel must be 'auto', 'tile', 'mfma', 'chain' or 'single'")
+        if kernel == "single" and x_dtype != "bf16":
+            raise ValueError("kernel 'single' (one bf16 copy of X) needs x_dtype='bf16'")
         self.kernel = kernel
         self.site_sizes = (("coefs", self.dim, None),)
 
     def attach(self, handle):
         lib = _lib.load()
-        kern = {"auto": 0, "tile": 1, "mfma": 2, "chain": 3}[self.kernel]
+        kern = {"auto": 0, "tile": 1, "mfma": 2, "chain": 3, "single": 4}[self.kernel]
         _lib.check(lib.nax_model_logistic_ex(handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N,
                                              self.dim, self.prior_scale,
                                              _lib.X_DTYPES[self.x_dtype], kern),

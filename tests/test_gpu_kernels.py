@@ -242,3 +242,39 @@ def test_logistic_mfma_bf16(lib, N, D, C):
                  + 2.0 ** -25 * np.abs(Xq).sum(0) + 1e-5 * (np.abs(gr).max() + 1.0))
         err = np.abs(g[c] - gr)
         assert np.all(err <= bound), (err / bound).max()
+
+
+@pytest.mark.parametrize("N, D, C", [(3001, 256, 128), (1000, 55, 40), (777, 26, 33),
+                                     (65, 32, 130), (4096, 200, 256), (50001, 256, 256),
+                                     (20011, 55, 64), (9001, 26, 40), (70003, 160, 1024),
+                                     (1, 1, 1), (5, 3, 2), (31, 33, 1), (33, 255, 3)])
+def test_logistic_mfma_bf16_single_copy(lib, N, D, C):
+    """Single-copy bf16 form (kernel 'single'): X^T R reads the bf16 copy of
+    X transposed (ds_read_b64_tr_b16) and R = sigmoid(l) - y is rounded to
+    bf16.  U as the two-copy form (1e-5 relative on the bf16-rounded X); dU
+    within |dg_d| <= 2^-8 sum_n |r_n x_nd| (bf16 unit roundoff of R) + fp32
+    accumulation."""
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import LogisticRegression
+    from scipy.special import expit
+    X, y, Z = _logistic_problem(N, D, C, 13 * N + D)
+    eng = Engine(LogisticRegression(X, y, prior_scale=0.7, x_dtype="bf16", kernel="single"), C,
+                 "NUTS", 0)
+    pe, g = _eval(eng, lib, Z)
+    Xq = _bf16_round(X)
+    for c in range(0, C, max(1, C // 9)):
+        zc = Z[c].astype(np.float64)
+        u, gr = om.logistic_potential_grad(zc, Xq, y, 0.7)
+        np.testing.assert_allclose(pe[c], u, rtol=1e-5)
+        r = expit(Xq @ zc) - y
+        bound = (2.0 ** -8 * (np.abs(r)[:, None] * np.abs(Xq)).sum(0)
+                 + 1e-5 * (np.abs(gr).max() + 1.0))
+        err = np.abs(g[c] - gr)
+        assert np.all(err <= bound), (err / bound).max()
+
+
+def test_single_copy_needs_bf16(lib):
+    from numpyro_amd.models import LogisticRegression
+    X, y, _ = _logistic_problem(10, 3, 1, 1)
+    with pytest.raises(ValueError, match="x_dtype='bf16'"):
+        LogisticRegression(X, y, kernel="single")
