@@ -107,8 +107,9 @@ def main():
     ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4"])
     ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
-    ap.add_argument("--bf16-kernel", default="two-copy", choices=["two-copy", "single"],
-                    help="bf16 storage: two X copies (fp16 residuals) or one (bf16 residuals)")
+    ap.add_argument("--bf16-kernel", default="single", choices=["single", "two-copy"],
+                    help="bf16 storage: one X copy (bf16 residuals, the library default) or "
+                         "two (fp16 residuals)")
     ap.add_argument("--rows", type=int, default=None, help="override N (smoke runs)")
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--fp32-steps", type=int, default=24,
@@ -141,7 +142,7 @@ def main():
 
     model = LogisticRegression(
         X, y, x_dtype=args.x_dtype,
-        kernel="single" if (args.x_dtype == "bf16" and args.bf16_kernel == "single") else "auto")
+        kernel=args.bf16_kernel if args.x_dtype == "bf16" else "auto")
     eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
@@ -321,13 +322,13 @@ def model_kernel_name(x_dtype, N, D, C, single=False):
 # HBM bytes per launch of the main kernel measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy: x2.0 / x1.0), see
 # profiles/logit_bf16_pipe_c5.md.  Only known for the measured workload.
-_MEASURED_TRAFFIC = {("bf16", 2_000_000, 256, 1024): 2_016_666 * 1024 + 33_024 * 1024}
+_MEASURED_TRAFFIC = {("single", 2_000_000, 256, 1024): (2 * 509_516 + 33_024) * 1024,
+                     ("two-copy", 2_000_000, 256, 1024): (2_016_666 + 33_024) * 1024}
 
 
 def traffic_bytes(args, N, D, C):
-    if args.x_dtype == "bf16" and args.bf16_kernel == "single":
-        return None  # not measured for the single-copy form
-    return _MEASURED_TRAFFIC.get((args.x_dtype, N, D, C))
+    form = args.bf16_kernel if args.x_dtype == "bf16" else "fp32"
+    return _MEASURED_TRAFFIC.get((form, N, D, C))
 
 
 if __name__ == "__main__":

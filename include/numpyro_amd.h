@@ -155,11 +155,12 @@ int nax_model_hier(nax_sampler* h, const float* d_x, const float* d_y, const int
  * n*d <= 2^20, d <= 64; otherwise the VALU tile kernel or, from 32 chains
  * up, the fp32 MFMA kernel, whichever the autotuner times faster), 1 = VALU
  * tile kernel, 2 = fp32 MFMA, 3 = one wavefront per chain.  bf16 storage
- * always uses the matrix cores: 0 keeps two copies of X (bf16 for X Z^T, an
- * fp16 copy for X^T R with fp16 residuals), 4 keeps one bf16 copy and reads
- * it transposed for X^T R with bf16 residuals (half the X bytes per
- * evaluation; the gradient's rounding grows from 2^-11 to 2^-8 per term).
- * 4 with fp32 storage is refused (NAX_EINVAL). */
+ * always uses the matrix cores: 0 (auto) or 4 keep one bf16 copy of X and
+ * read it transposed for X^T R with bf16 residuals; 2 keeps a second, fp16
+ * copy for X^T R with fp16 residuals (twice the X bytes per evaluation, a
+ * gradient rounding of 2^-11 instead of 2^-8 per term, and max|X| < 60000).
+ * Other kernel values with bf16 storage, and 4 with fp32 storage, are
+ * refused (NAX_EINVAL). */
 int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                           int32_t d, double prior_scale, int32_t x_dtype, int32_t kernel);
 

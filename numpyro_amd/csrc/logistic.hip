@@ -253,13 +253,18 @@ Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
 
 // fuse_variant: 0 = auto (matrix cores when num_chains >= 32), 1 = VALU tile
 // kernel, 2 = matrix cores, 3 = one wavefront per chain.  bf16 always runs on
-// the matrix cores; with fuse_variant 4 it keeps one X copy (bf16
-// residuals in X^T R).
+// the matrix cores: 0 or 4 keep one X copy (bf16 residuals in X^T R), 2 keeps
+// a second, fp16 copy for X^T R with fp16 residuals.
 Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
                          int x_dtype, int fuse_variant, int num_chains, hipStream_t s,
                          std::string* err) {
-  if (x_dtype == NAX_X_BF16)
-    return make_logistic_mfma(x, y, n, d, prior_scale, true, fuse_variant == 4, s, err);
+  if (x_dtype == NAX_X_BF16) {
+    if (fuse_variant != 0 && fuse_variant != 2 && fuse_variant != 4) {
+      *err = "bf16 storage takes kernel 0 or 4 (one copy of X) or 2 (two copies)";
+      return nullptr;
+    }
+    return make_logistic_mfma(x, y, n, d, prior_scale, true, fuse_variant != 2, s, err);
+  }
   if (fuse_variant == 4) {
     *err = "kernel 4 (single-copy bf16) needs bf16 storage";
     return nullptr;

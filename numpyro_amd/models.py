@@ -58,7 +58,8 @@ class LogisticRegression(Model):
 
     ``X`` [N, D] and ``y`` [N] are moved to the GPU as float32.  ``x_dtype``
     picks the storage the gradient kernels read: ``"fp32"`` (plain) or
-    ``"bf16"`` (MFMA).  ``"fp8"`` is refused: both operands of an fp8 MFMA are
+    ``"bf16"`` (MFMA; by default one bf16 copy of X with bf16 residuals in
+    X^T R, ``kernel="two-copy"`` adds an fp16 copy for fp16 residuals).  ``"fp8"`` is refused: both operands of an fp8 MFMA are
     8-bit, so the positions would need a three-term e4m3 split (as costly as
     the two bf16 products) and the residuals of X^T R would keep 3 mantissa
     bits."""
@@ -76,16 +77,20 @@ class LogisticRegression(Model):
         if x_dtype not in _lib.X_DTYPES:
             raise ValueError(f"unknown x_dtype {x_dtype!r}")
         self.x_dtype = x_dtype
-        if kernel not in ("auto", "tile", "mfma", "chain", "single"):
-            raise ValueError("kernel must be 'auto', 'tile', 'mfma', 'chain' or 'single'")
-        if kernel == "single" and x_dtype != "bf16":
-            raise ValueError("kernel 'single' (one bf16 copy of X) needs x_dtype='bf16'")
+        if kernel not in ("auto", "tile", "mfma", "chain", "single", "two-copy"):
+            raise ValueError("kernel must be 'auto', 'tile', 'mfma', 'chain', 'single' or "
+                             "'two-copy'")
+        if kernel in ("single", "two-copy") and x_dtype != "bf16":
+            raise ValueError(f"kernel {kernel!r} (a bf16 form) needs x_dtype='bf16'")
+        if x_dtype == "bf16" and kernel in ("tile", "chain"):
+            raise ValueError("bf16 storage runs on the matrix cores: kernel 'auto', 'single' "
+                             "or 'two-copy'")
         self.kernel = kernel
         self.site_sizes = (("coefs", self.dim, None),)
 
     def attach(self, handle):
         lib = _lib.load()
-        kern = {"auto": 0, "tile": 1, "mfma": 2, "chain": 3, "single": 4}[self.kernel]
+        kern = {"auto": 0, "tile": 1, "mfma": 2, "chain": 3, "single": 4, "two-copy": 2}[self.kernel]
         _lib.check(lib.nax_model_logistic_ex(handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N,
                                              self.dim, self.prior_scale,
                                              _lib.X_DTYPES[self.x_dtype], kern),

@@ -22,7 +22,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c4", "c3", "c3nc"])
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
-    ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "mfma", "single"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "mfma", "single", "two-copy"])
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--chains", type=int, default=None)

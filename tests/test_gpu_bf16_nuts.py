@@ -21,7 +21,7 @@ def test_bf16_matches_fp32_posterior():
     y = (rng.uniform(size=N) < 1 / (1 + np.exp(-X @ beta))).astype(np.float32)
     res = {}
     for xd in ("fp32", "bf16"):
-        m = LogisticRegression(X, y, x_dtype=xd, kernel="mfma")
+        m = LogisticRegression(X, y, x_dtype=xd, kernel="mfma" if xd == "fp32" else "auto")
         mcmc = MCMC(NUTS(m), num_warmup=300, num_samples=300, num_chains=C)
         mcmc.run(random.PRNGKey(0), init_params=torch.zeros(C, D))
         z = mcmc.get_samples(group_by_chain=True)["coefs"].cpu().numpy().astype(np.float64)

@@ -219,7 +219,8 @@ def test_logistic_mfma_fp32(lib, N, D, C):
                                      # one padded row tile, D = 1, one chain, ragged D
                                      (1, 1, 1), (5, 3, 2), (31, 33, 1), (33, 255, 3)])
 def test_logistic_mfma_bf16(lib, N, D, C):
-    """BF16 storage defines the model with X rounded to bf16.  Against the
+    """Two-copy BF16 form (kernel 'two-copy').  BF16 storage defines the model
+    with X rounded to bf16.  Against the
     float64 oracle on that X: U to 1e-5 relative (positions enter as a bf16
     hi+lo pair, ~2^-17 relative); dU within the rounding bound of the fp16
     R = sigmoid(l) - y of the second GEMM: |dg_d| <= 2^-11 sum_n |r_n x_nd|
@@ -230,7 +231,8 @@ def test_logistic_mfma_bf16(lib, N, D, C):
     from numpyro_amd.models import LogisticRegression
     from scipy.special import expit
     X, y, Z = _logistic_problem(N, D, C, 11 * N + D)
-    eng = Engine(LogisticRegression(X, y, prior_scale=0.7, x_dtype="bf16"), C, "NUTS", 0)
+    eng = Engine(LogisticRegression(X, y, prior_scale=0.7, x_dtype="bf16", kernel="two-copy"), C,
+                 "NUTS", 0)
     pe, g = _eval(eng, lib, Z)
     Xq = _bf16_round(X)
     for c in range(0, C, max(1, C // 9)):
@@ -249,7 +251,7 @@ def test_logistic_mfma_bf16(lib, N, D, C):
                                      (20011, 55, 64), (9001, 26, 40), (70003, 160, 1024),
                                      (1, 1, 1), (5, 3, 2), (31, 33, 1), (33, 255, 3)])
 def test_logistic_mfma_bf16_single_copy(lib, N, D, C):
-    """Single-copy bf16 form (kernel 'single'): X^T R reads the bf16 copy of
+    """Single-copy bf16 form (kernel 'single', the bf16 default): X^T R reads the bf16 copy of
     X transposed (ds_read_b64_tr_b16) and R = sigmoid(l) - y is rounded to
     bf16.  U as the two-copy form (1e-5 relative on the bf16-rounded X); dU
     within |dg_d| <= 2^-8 sum_n |r_n x_nd| (bf16 unit roundoff of R) + fp32
