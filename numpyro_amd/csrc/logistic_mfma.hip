@@ -62,7 +62,11 @@ __device__ __forceinline__ void logit_terms(float l, float y, float& r, float& u
 
 // ---- fragment-order preparation -------------------------------------------------
 // KS = K per MFMA (16 for bf16, 2 for fp32); Dp = D rounded up to 32.
-template <typename T, int KS>
+// SWZ (single-copy bf16 form): within each 64-fragment block the second
+// half's rows are stored XOR 4 (position p holds lane p ^ 4 for p >= 32), so
+// the transposed GEMM2 reads of a row's two 8-column halves fall on
+// different LDS banks.
+template <typename T, int KS, bool SWZ = false>
 __global__ void prep_a1(const float* __restrict__ X, int64_t N, int D, int Dp, int64_t ntile,
                         T* __restrict__ out) {
   const int KK = Dp / KS;
@@ -72,6 +76,7 @@ __global__ void prep_a1(const float* __restrict__ X, int64_t N, int D, int Dp, i
     int j = (int)(i % (KS / 2));
     int64_t r = i / (KS / 2);
     int lane = (int)(r % 64);
+    if (SWZ) lane ^= (lane >> 5) << 2;
     r /= 64;
     int kk = (int)(r % KK);
     int64_t t = r / KK;
@@ -479,9 +484,15 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   // element q % 4 of the 8-byte chunks addressed by lanes q / 4 + 4 k
   // (k = 0..3), so lane q addresses row (q / 4) of its group's 4-row block at
   // columns 4 (q % 4) .. 4 (q % 4) + 3 of its group's 16-column half.
+  // The single-copy A1 stores the second half's rows XOR 4 (prep_a1<SWZ>):
+  // GEMM1 lane l reads its fragment at position lsw, and the transposed
+  // reads address row r of half h at position 32 h + (r ^ 4 h).
+  const int lsw = SINGLE ? (l ^ ((l >> 5) << 2)) : l;
   const int jl = l & 15, gl = l >> 4;
+  const int hb = (jl & 3) >> 1;
   const int tr_off =
-      (((gl & 1) * 64 + ((jl & 3) >> 1) * 32 + 4 * (gl >> 1) + (jl >> 2)) * 16) + 8 * (jl & 1);
+      (((gl & 1) * 64 + hb * 32 + 4 * ((gl >> 1) ^ (SINGLE ? hb : 0)) + (jl >> 2)) * 16) +
+      8 * (jl & 1);
   auto bfrag = [&](int64_t i, int m2, int dt) -> rfrag {
     if constexpr (SINGLE) {
       typedef short v4s __attribute__((ext_vector_type(4)));
@@ -508,7 +519,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     for (int v = 0; v < 16; ++v) L[v] = 0.0f;
 #pragma unroll
     for (int kk = 0; kk < KK; ++kk) {
-      const bf16x8 af = la[kk * 64 + l];
+      const bf16x8 af = la[kk * 64 + lsw];
       L = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zh[kk], L, 0, 0, 0);
       L = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zlo[kk], L, 0, 0, 0);
     }
@@ -565,7 +576,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     for (int e = 0; e < 16; ++e) {
 #pragma unroll
       for (int kk = e * KK / 16; kk < (e + 1) * KK / 16; ++kk) {
-        const bf16x8 af = la[kk * 64 + l];
+        const bf16x8 af = la[kk * 64 + lsw];
         Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zh[kk], Ln, 0, 0, 0);
         Ln = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, zlo[kk], Ln, 0, 0, 0);
       }
@@ -596,14 +607,14 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
     // epilogue of element e; fragments for chunk e + 1 are read from LDS one
     // chunk ahead.  Scheduling barriers keep each chunk's VALU work next to
     // its MFMAs so that it runs in their shadow.
-    bf16x8 af = la[l];
+    bf16x8 af = la[lsw];
     rfrag bf = bfrag(i - 1, 0, 0);
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       bf16x8 af_n;
       rfrag bf_n;
       if (e + 1 < 16) {
-        af_n = la[(e + 1) * 64 + l];
+        af_n = la[(e + 1) * 64 + lsw];
         const int jn = e + 1, m2n = jn / DT, dtn = jn % DT;
         bf_n = bfrag(i - 1, m2n, dtn);
       }
@@ -746,7 +757,7 @@ struct LogisticMfma : Potential {
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) num_cu = prop.multiProcessorCount;
     const int blocks = 4096;
     if (bf16 && single) {
-      hipLaunchKernelGGL((prep_a1<__bf16, 16>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)a1);
+      hipLaunchKernelGGL((prep_a1<__bf16, 16, true>), dim3(blocks), dim3(256), 0, s, x, n, d, dp, ntile, (__bf16*)a1);
     } else if (bf16) {
       unsigned* dmax = nullptr;
       e = hipMallocAsync((void**)&dmax, sizeof(unsigned), s);
