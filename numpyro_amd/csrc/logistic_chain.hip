@@ -57,13 +57,34 @@ struct LogisticChainModel {
     }
     double pr = 0.0;
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) {
-      if (k < d) {
-        const float s = wave_sum(gp[k]);
-        if (lane == (k & (NAX_WAVE - 1))) g[k] = s + zr[k] * inv_s2;
-        pr += 0.5 * (double)zr[k] * zr[k] * inv_s2;
+    for (int k = 0; k < DMAX; ++k)
+      if (k < d) pr += 0.5 * (double)zr[k] * zr[k] * inv_s2;
+    // Transpose-reduce of the DMAX partial gradients over the 64 lanes: the
+    // step with shuffle distance `off` keeps, in each lane, the half of its
+    // coordinates selected by lane bit `off` and adds the partner lane's
+    // copy of that half.  After log2(DMAX) steps lane l holds coordinate
+    // k = l >> (6 - log2 DMAX), summed over the lanes that agree with it on
+    // the bits not yet used; the remaining xor steps finish the sum.
+    // 31 (DMAX 32) or 63 (DMAX 64) shuffles instead of 6 per coordinate.
+    int n = DMAX, kbit = 0;
+#pragma unroll
+    for (int off = NAX_WAVE / 2; n > 1; off >>= 1, n >>= 1) {
+      const int half = n / 2;
+      const bool upper = (lane & off) != 0;
+#pragma unroll
+      for (int i = 0; i < half; ++i) {
+        const float keep = upper ? gp[half + i] : gp[i];
+        const float give = upper ? gp[i] : gp[half + i];
+        gp[i] = keep + __shfl_xor(give, off, NAX_WAVE);
       }
+      kbit = off;
     }
+#pragma unroll
+    for (int off = kbit >> 1; off > 0; off >>= 1) gp[0] += __shfl_xor(gp[0], off, NAX_WAVE);
+    constexpr int shift = DMAX == 64 ? 0 : (DMAX == 32 ? 1 : -1);
+    static_assert(shift >= 0, "DMAX is 32 or 64");
+    const int k = lane >> shift;
+    if ((lane & ((1 << shift) - 1)) == 0 && k < d) g[k] = gp[0] + z[k] * inv_s2;
     return wave_sum(u) + pr + log_norm;
   }
 };
