@@ -354,8 +354,14 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // t+1), so each has its own 4-slot ring: iteration i issues A1 of tile i+4,
 // B2 of tile i+2 and y of tile i+3 into the slots freed by iteration i-1,
 // and then waits only for the loads issued two iterations earlier -- every
-// load has three iterations (~4 us) to arrive.  GEMM2 runs in fp16: X (bf16
+// load has three iterations (~4 us) to arrive.
+//
+// SINGLE = false (two-copy form): GEMM2 runs in fp16 on the B2 copy: X (bf16
 // values) is exact in fp16 and R gets 11 mantissa bits in a single MFMA.
+// SINGLE = true (single-copy form, the default): there is no B2; GEMM2 runs
+// in bf16 and reads its X operand transposed from the A1 tile, which
+// therefore stays in LDS until iteration t+1 (8-slot A1 ring), and R gets
+// 8 mantissa bits.
 template <int DP, bool SINGLE>
 __global__ void __launch_bounds__(256, 1)
 logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
