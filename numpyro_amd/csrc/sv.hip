@@ -75,7 +75,10 @@ struct SVModel {
     const double nu = exp(u_nu), sig = exp(u_sig);
     const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
     const float inv_s2 = (float)(1.0 / (sig * sig));
-    double U = 0.0, S2 = 0.0, dnu = 0.0;
+    const float inv_nu = 1.0f / nuf;
+    // dU/dnu's data term sum_t 0.5 a_t - 0.5 (nu+1) b_t / nu: sum a and b per
+    // lane, scale once (nu is constant within an evaluation)
+    double U = 0.0, S2 = 0.0, asum = 0.0, bsum = 0.0;
 #pragma unroll 4
     for (int t = lane; t < T; t += NAX_WAVE) {
       const float s = z[1 + t];
@@ -83,17 +86,17 @@ struct SVModel {
       const float dnext = (t + 1 < T) ? (z[2 + t] - s) : 0.0f;
       const float rr = r[t];
       const float y2 = rr * rr * __expf(-2.0f * s);
-      const float q = y2 / nuf;
-      const float a = log1pf(q);
+      const float a = log1pf(y2 * inv_nu);
       const float b = y2 / (nuf + y2);
       U += (double)(0.5f * dprev * dprev * inv_s2) + 0.5 * (double)nu1 * (double)a + (double)s;
       S2 += (double)dprev * (double)dprev;
-      dnu += 0.5 * (double)a - 0.5 * (double)nu1 * (double)b / nu;
+      asum += (double)a;
+      bsum += (double)b;
       g[1 + t] = (dprev - dnext) * inv_s2 + 1.0f - nu1 * b;
     }
     U = wave_sum(U);
     S2 = wave_sum(S2);
-    dnu = wave_sum(dnu);
+    const double dnu = 0.5 * wave_sum(asum) - 0.5 * (double)nu1 * wave_sum(bsum) / nu;
     double uc, dnc;
     studentt_consts(nu, uc, dnc);
     U += uc + T * (u_sig + 0.5 * kLog2Pi)             // GRW normaliser
@@ -121,7 +124,8 @@ struct SVModel {
     for (int t = t0; t < t1; ++t) seg += (double)zeta[t];
     const double cp = wave_inclusive_scan(seg, lane) - seg;
     // pass 2: StudentT terms and dU/ds_t = 1 - (nu+1) y^2 / (nu + y^2)
-    double U = 0.0, dnu = 0.0, A = 0.0, Asc = 0.0, Z2 = 0.0;
+    const float inv_nu = 1.0f / nuf;
+    double U = 0.0, asum = 0.0, bsum = 0.0, A = 0.0, Asc = 0.0, Z2 = 0.0;
     double c = cp;
 #pragma unroll 4
     for (int t = t0; t < t1; ++t) {
@@ -130,11 +134,12 @@ struct SVModel {
       const float s = (float)(sig * c);
       const float rr = r[t];
       const float y2 = rr * rr * __expf(-2.0f * s);
-      const float a = log1pf(y2 / nuf);
+      const float a = log1pf(y2 * inv_nu);
       const float b = y2 / (nuf + y2);
       const float ds = 1.0f - nu1 * b;
       U += 0.5 * (double)nu1 * (double)a + (double)s;
-      dnu += 0.5 * (double)a - 0.5 * (double)nu1 * (double)b / nu;
+      asum += (double)a;
+      bsum += (double)b;
       A += (double)ds;
       Asc += (double)ds * c;
       Z2 += (double)zt * (double)zt;
@@ -156,7 +161,7 @@ struct SVModel {
       P += (double)ds;
     }
     U = wave_sum(U) + 0.5 * wave_sum(Z2);
-    dnu = wave_sum(dnu);
+    const double dnu = 0.5 * wave_sum(asum) - 0.5 * (double)nu1 * wave_sum(bsum) / nu;
     const double asc = wave_sum(Asc);
     double uc, dnc;
     studentt_consts(nu, uc, dnc);
