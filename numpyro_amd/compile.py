@@ -10,8 +10,9 @@ resulting site graph is matched against the model families whose
 log-density and adjoint exist as fused HIP kernels:
 
 * logistic regression: ``b ~ Normal(0, s)`` [D] (any batch/event split of the
-  D entries), ``y ~ Bernoulli(logits = X @ b)`` (or ``sum(b * X, -1)``) [N]
-  -> ``models.LogisticRegression`` (``examples/covtype.py:66-71``,
+  D entries), ``y ~ Bernoulli(logits = X @ b)`` (or ``sum(b * X, -1)``,
+  ``b @ X.T``), optionally ``+ a`` with an intercept ``a ~ Normal(0, s)``
+  [N] -> ``models.LogisticRegression`` (the intercept as a column of ones) (``examples/covtype.py:66-71``,
   ``test/infer/test_mcmc.py:104-168``);
 * stochastic volatility: ``sigma ~ Exponential(a)``, ``s ~
   GaussianRandomWalk(sigma, T)``, ``nu ~ Exponential(b)``, ``r ~ StudentT(nu,
@@ -277,6 +278,10 @@ def _linear(v):
             X = _cval(a)
             if X.ndim == 2 and len(b.shape) == 1:
                 return X, b
+        if _is_const(b) and isinstance(a, Sym) and a.op != "const":  # b @ X^T
+            Xt = _cval(b)
+            if Xt.ndim == 2 and len(a.shape) == 1:
+                return Xt.T, a
         raise _NoMatch
     if v.op == "sum":
         dims = v.args[1:] if len(v.args) > 1 else (v.kwargs.get("dim", v.kwargs.get("axis")),)
@@ -305,27 +310,54 @@ def _check_obs_shape(site, n):
         raise _NoMatch
 
 
-def _match_logistic(lat, obs):
-    if len(lat) != 1 or len(obs) != 1:
-        raise _NoMatch
-    (bname, bsite), = lat.items()
-    (oname, osite), = obs.items()
-    pd = bsite["fn"].base
-    if not isinstance(pd, dist.Normal) or len(bsite["shape"]) != 1:
+def _normal_scale(site, name):
+    pd = site["fn"].base
+    if not isinstance(pd, dist.Normal):
         raise _NoMatch
     _uniform_scalar(pd.loc, 0.0)
     s = _uniform_scalar(pd.scale)
     if not s > 0:
-        raise ValueError(f"site {bname!r}: Normal scale must be positive, got {s}")
+        raise ValueError(f"site {name!r}: Normal scale must be positive, got {s}")
+    return s
+
+
+def _match_logistic(lat, obs):
+    """b ~ Normal(0, s) [D], optionally a ~ Normal(0, s) scalar intercept
+    with the same s, y ~ Bernoulli(logits = X b [+ a]).  The intercept is the
+    kernel's last coefficient on an appended column of ones."""
+    if len(lat) not in (1, 2) or len(obs) != 1:
+        raise _NoMatch
+    (oname, osite), = obs.items()
     od = osite["fn"].base
     if not isinstance(od, dist.BernoulliLogits):
         raise _NoMatch
-    X, e = _linear(od.logits)
-    if _latent(e) != bname or X.shape[1] != bsite["shape"][0]:
+    logits, aname = od.logits, None
+    if len(lat) == 2:
+        if not (isinstance(logits, Sym) and logits.op == "add"):
+            raise _NoMatch
+        p, q = logits.args
+        if isinstance(p, Sym) and p.op == "latent":
+            p, q = q, p
+        aname = _latent(q)
+        if aname not in lat or _flat_size(lat[aname]["shape"]) != 1:
+            raise _NoMatch
+        logits = p
+    X, e = _linear(logits)
+    bname = _latent(e)
+    if bname not in lat or bname == aname or len(lat[bname]["shape"]) != 1 or \
+            X.shape[1] != lat[bname]["shape"][0]:
         raise _NoMatch
+    s = _normal_scale(lat[bname], bname)
     _check_obs_shape(osite, X.shape[0])
+    D = X.shape[1]
+    sites = ((bname, D, None),)
+    if aname is not None:
+        if _normal_scale(lat[aname], aname) != s:
+            raise _NoMatch  # one prior scale for all coefficients
+        X = torch.cat([X, torch.ones((X.shape[0], 1), dtype=X.dtype, device=X.device)], 1)
+        sites = sites + ((aname, 1, None),)
     return "logistic", dict(X=X, y=_cval(osite["value"]).reshape(-1), prior_scale=s,
-                            sites=((bname, bsite["shape"][0], None),))
+                            sites=sites)
 
 
 def _match_sv(lat, obs):
@@ -436,7 +468,8 @@ _FORMS = (_match_logistic, _match_sv, _match_hier)
 
 _REFUSAL = (
     "the model does not match a form with fused HIP kernels: logistic regression "
-    "(b ~ Normal(0, s) [D]; y ~ Bernoulli(logits=X @ b)), stochastic volatility "
+    "(b ~ Normal(0, s) [D], optional intercept a ~ Normal(0, s); "
+    "y ~ Bernoulli(logits=X @ b [+ a])), stochastic volatility "
     "(Exponential sigma and nu, GaussianRandomWalk s or s = sigma * cumsum(zeta) with "
     "zeta ~ Normal(0, 1), StudentT(nu, 0, exp(s)) returns) or non-centred hierarchical "
     "logistic regression (eta ~ Normal(0, 1) [D <= 64], mu ~ Normal(0, 1), "

@@ -223,3 +223,33 @@ def test_invalid_prior_scale_is_an_error():
 
     with pytest.raises(ValueError, match="scale must be positive"):
         match_model(model)
+
+
+def test_logistic_with_intercept():
+    X, y = _logistic_data(120, 4, 12)
+
+    def model(X, y):
+        a = npa.sample("a", dist.Normal(0.0, 2.0))
+        b = npa.sample("b", dist.Normal(0.0, 2.0).expand([4]).to_event(1))
+        npa.sample("y", dist.Bernoulli(logits=b @ X.T + a), obs=y)
+
+    kind, p, _ = match_model(model, X, y)
+    assert kind == "logistic" and p["prior_scale"] == 2.0
+    assert p["sites"] == (("b", 4, None), ("a", 1, None))
+    assert p["X"].shape == (120, 5) and bool((p["X"][:, 4] == 1).all())
+    z = np.random.default_rng(13).normal(size=5)
+    np.testing.assert_allclose(_interpreted_potential(model, (X, y), p["sites"], z),
+                               om.logistic_potential(z, p["X"].numpy(), y.numpy(), 2.0),
+                               rtol=1e-10)
+
+
+def test_intercept_with_other_scale_is_refused():
+    X, y = _logistic_data(50, 3)
+
+    def model():
+        a = npa.sample("a", dist.Normal(0.0, 10.0))
+        b = npa.sample("b", dist.Normal(0.0, 1.0).expand([3]))
+        npa.sample("y", dist.Bernoulli(logits=X @ b + a), obs=y)
+
+    with pytest.raises(NotImplementedError):
+        match_model(model)

@@ -137,3 +137,27 @@ def test_compiled_sv_and_hier_match_builtin():
     sa, sb = a.get_samples(), b.get_samples()
     assert torch.equal(sa["mu"], sb["mu"]) and torch.equal(sa["tau"], sb["tau"])
     assert torch.equal(sa["eta"], sb["eta"][:, :5])
+
+
+def test_compiled_intercept_matches_augmented_builtin():
+    import numpyro_amd as npa
+    from numpyro_amd import distributions as dist
+    from numpyro_amd import random
+    from numpyro_amd.infer import MCMC, NUTS
+    from numpyro_amd.models import LogisticRegression
+    X, y = _logistic_data(1500, 3, 4)
+
+    def model(X, y):
+        b = npa.sample("b", dist.Normal(0.0, 1.5).expand([3]).to_event(1))
+        a = npa.sample("a", dist.Normal(0.0, 1.5))
+        npa.sample("y", dist.Bernoulli(logits=X @ b + a), obs=y)
+
+    m = MCMC(NUTS(model), num_warmup=100, num_samples=100, num_chains=2)
+    m.run(random.PRNGKey(8), X, y)
+    Xa = torch.cat([X, torch.ones(X.shape[0], 1)], 1)
+    r = MCMC(NUTS(LogisticRegression(Xa, y, prior_scale=1.5)), num_warmup=100, num_samples=100,
+             num_chains=2)
+    r.run(random.PRNGKey(8))
+    s, t = m.get_samples(), r.get_samples()["coefs"]
+    assert set(s) == {"a", "b"}
+    assert torch.equal(s["b"], t[:, :3]) and torch.equal(s["a"], t[:, 3])
