@@ -104,7 +104,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=150)
-    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4"])
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4", "ecs"])
+    ap.add_argument("--subsample", type=int, default=1000, help="ecs: subsample size m")
+    ap.add_argument("--num-blocks", type=int, default=100, help="ecs: Gibbs blocks")
     ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--bf16-kernel", default="single", choices=["single", "two-copy"],
@@ -132,6 +134,8 @@ def main():
         return bench_sv(args)
     if args.config == "c4":
         return bench_hier(args)
+    if args.config == "ecs":
+        return bench_ecs(args)
     if args.config in ("c1", "c2") and args.x_dtype == "bf16" and args.config == "c1":
         args.x_dtype = "fp32"  # 4 chains: the VALU tile kernel
     X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
@@ -300,6 +304,61 @@ def bench_hier(args):
             "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
             "algorithmic_bytes_per_sweep": model.algorithmic_bytes(C),
             "hbm_GBps_at_grad_rate": model.algorithmic_bytes(C) * ge / C / wall / 1e9}
+    log(json.dumps(line))
+    print(json.dumps(line), flush=True)
+
+
+def bench_ecs(args):
+    """HMCECS on the covtype-sized logistic design (examples/covtype.py:154-165:
+    subsample_size 1000, num_blocks 100, Taylor proxy of degree 2): the C5
+    data (N=2M, D=256, fp32 X) with 1024 chains, each on its own subsample.
+    The proxy's reference point is the data-generating coefficient vector
+    (the example uses a MAP estimate)."""
+    import torch
+    from numpyro_amd import _lib
+    from numpyro_amd import random as nrandom
+    from numpyro_amd import workloads
+    from numpyro_amd.models import SubsampledLogisticRegression
+    X, y, beta, cfg = workloads.make("c5", N=args.rows, chains=args.chains)
+    N, D = X.shape
+    C = args.chains or 1024
+    m, nb = args.subsample, args.num_blocks
+    t0 = time.perf_counter()
+    model = SubsampledLogisticRegression(X, y, m, num_blocks=nb, proxy_degree=2,
+                                         reference=torch.as_tensor(beta))
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
+    kd = torch.as_tensor(np.asarray(keys, np.uint32).view(np.int32)).cuda().contiguous()
+    model.keys = kd.clone()
+    model.idx = torch.empty((C, m), dtype=torch.int32, device="cuda")
+    _lib.check(_lib.load().nax_randint(_lib.ptr(kd), C, m, 0, N, int(nrandom.PARTITIONABLE),
+                                       _lib.ptr(model.idx)), "randint")
+    # init_to_value(values=ref_params), as the example's inner kernel
+    z0 = beta.to(torch.float32).reshape(1, D).repeat(C, 1).contiguous()
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    setup_s = time.perf_counter() - t0 - wall - warm_s
+    mess, rhat = min_ess(out["z"])
+    ge = eng.grad_evals
+    per_eval = model.algorithmic_bytes(1)
+    ach = ge * per_eval / (kms / 1e3) / 1e9 if kms > 0 else None
+    peak = 8000.0
+    line = {"metric": "min-ESS/sec + grad evals/sec, 1024-chain HMCECS logistic NUTS",
+            "value": ge / wall, "unit": "grad evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic",
+            "config": {"workload": f"HMCECS logistic NUTS: N={N}, D={D}, chains={C}, "
+                                   f"subsample {m}, {nb} blocks, Taylor proxy degree 2",
+                       "global_batch": C, "seq_len": m, "parallelism": "chains"},
+            "min_ess_per_s": mess / wall, "min_ess": mess, "max_rhat": rhat,
+            "mean_tree_size": float(out["num_steps"].float().mean()),
+            "mean_accept_prob": float(out["accept_prob"].float().mean()),
+            "divergences": int(out["diverging"].sum()),
+            "gibbs_accept_prob_mean": float(eng.gibbs_accept.mean()),
+            "warmup_s": warm_s, "setup_s": setup_s,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": peak,
+                         "unit": "GB/s", "frac": ach / peak if ach else None, "traffic": None,
+                         "kernel": "ecs_eval_kernel", "kernel_avg_ms": kms / max(kn, 1),
+                         "launches": kn, "algorithmic_bytes_per_chain_eval": per_eval}}
     log(json.dumps(line))
     print(json.dumps(line), flush=True)
 

@@ -164,6 +164,28 @@ int nax_model_hier(nax_sampler* h, const float* d_x, const float* d_y, const int
 int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                           int32_t d, double prior_scale, int32_t x_dtype, int32_t kernel);
 
+/* Subsampled logistic regression for HMCECS (numpyro/infer/hmc_gibbs.py:502-,
+ * the covtype model of examples/covtype.py:66-71 with subsample_size, run as
+ * in examples/covtype.py:154-165).  Each chain's potential uses its own
+ * subsample of subsample_size rows, d_idx [C, subsample_size] int32 (caller-
+ * owned, initialised by the caller, updated by nax_ecs_gibbs).  proxy_degree
+ * 0: the plate's n/m scaling of the subsample log-likelihood; 1 or 2: the
+ * difference estimator with a Taylor proxy around d_ref [d] fp64
+ * (numpyro/contrib/ecs_proxies.py:23-50, 95-250).  d_keys [C, 2] are the
+ * HMCECS state keys (HMCECSState.rng_key), advanced by nax_ecs_gibbs.
+ * subsample_size <= 4096, dim <= 1024. */
+int nax_model_logistic_ecs(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
+                           int32_t d, double prior_scale, int32_t subsample_size,
+                           int32_t num_blocks, int32_t proxy_degree, const double* d_ref,
+                           int32_t* d_idx, uint32_t* d_keys);
+
+/* One HMCECS Gibbs update of every chain's subsample (hmc_gibbs.py:640-672,
+ * block update ecs_proxies.py:58-92): a block of indices is redrawn, U and
+ * dU at the chain's current position are evaluated with the proposal, and
+ * the proposal is accepted with probability min(1, exp(U - U_new)).  Call it
+ * before each nax_run of one iteration.  d_accept_prob [C] fp32 or NULL. */
+int nax_ecs_gibbs(nax_sampler* h, float* d_accept_prob);
+
 /* Potential supplied by the caller (a potential_fn, hmc.py:127-130): the
  * engine asks for U and dU at d_z_request, the caller writes them into the
  * gradient buffers, see nax_advance. */
@@ -221,6 +243,13 @@ int nax_threefry2x32(const uint32_t* d_key /*2*/, const uint32_t* d_x0, const ui
                      uint32_t* d_out0, uint32_t* d_out1, int64_t n);
 int nax_random_split(const uint32_t* d_key /*2*/, int32_t num, int32_t partitionable,
                      uint32_t* d_out /*[num,2]*/);
+/* random.randint(keys[c], (n,), lo, hi) for nk keys (jax.random.randint's
+ * two-draw construction, as ecs_proxies.py:58-71 consumes it): d_out [nk, n]. */
+int nax_randint(const uint32_t* d_keys /*[nk,2]*/, int32_t nk, int32_t n, int32_t lo, int32_t hi,
+                int32_t partitionable, int32_t* d_out);
+/* random.split(keys[c], num) for nk keys: d_out [nk, num, 2]. */
+int nax_split_keys(const uint32_t* d_keys /*[nk,2]*/, int32_t nk, int32_t num,
+                   int32_t partitionable, uint32_t* d_out);
 int nax_random_normal(const uint32_t* d_key /*2*/, int32_t n, int32_t partitionable,
                       float* d_out);
 

@@ -18,5 +18,5 @@ or, with a model function compiled onto the HIP kernels,
 __version__ = "0.1.0"
 
 from . import diagnostics, distributions, random  # noqa: F401
-from .primitives import deterministic, plate, sample  # noqa: F401
+from .primitives import deterministic, plate, sample, subsample  # noqa: F401
 from . import compile  # noqa: F401,E402

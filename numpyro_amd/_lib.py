@@ -65,6 +65,13 @@ SIGNATURES = {
     "nax_model_hier": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_int32, C.c_int32]),
     "nax_model_external": (C.c_int, [C.c_void_p]),
+    "nax_model_logistic_ecs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                         C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nax_ecs_gibbs": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nax_randint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                              C.c_void_p]),
+    "nax_split_keys": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "nax_init": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nax_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(nax_collect),
                           C.POINTER(C.c_int64)]),

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnumpyro_amd.so")
 SOURCES = ["sampler.hip", "logistic.hip", "logistic_mfma.hip", "logistic_chain.hip", "sv.hip",
-           "hier.hip", "diagnostics.hip"]
+           "hier.hip", "diagnostics.hip", "ecs.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -486,13 +486,26 @@ def match_model(model, *args, **kwargs):
     for k, v in obs.items():
         if isinstance(v["value"], Sym):
             raise NotImplementedError(f"observed site {k!r}: the observation must be data")
+    sub = {p.name: (p.size, p.subsample_size) for v in obs.values() for p in v["plates"]
+           if p.subsample_size < p.size}
     for form in _FORMS:
         try:
             kind, params = form(lat, obs)
         except _NoMatch:
             continue
+        if sub:
+            (pname, (n, m)), = sub.items() if len(sub) == 1 else _many_subsample_plates(sub)
+            if kind != "logistic" or params["X"].shape[0] != n:
+                raise NotImplementedError(
+                    "subsample plates (HMCECS) are supported for logistic regression, the "
+                    "plate running over the rows of X")
+            params["subsample"] = (pname, n, m)
         return kind, params, tr
     raise NotImplementedError(_REFUSAL)
+
+
+def _many_subsample_plates(sub):
+    raise NotImplementedError(f"more than one subsample plate: {sorted(sub)}")
 
 
 # ---- building the engine model ----------------------------------------------

@@ -1,0 +1,510 @@
+// HMCECS for logistic regression: the subsampled potential with the
+// difference estimator (Taylor proxy of degree 1 or 2 around reference
+// coefficients) or the plain n/m-scaled subsample log-likelihood, and the
+// Gibbs block update of each chain's subsample indices.
+//
+//   estimator    numpyro/contrib/ecs_proxies.py:23-50
+//   block update numpyro/contrib/ecs_proxies.py:58-92
+//   Taylor proxy numpyro/contrib/ecs_proxies.py:95-250
+//   Gibbs step   numpyro/infer/hmc_gibbs.py:640-681
+//   model        examples/covtype.py:66-71 with subsample_size,
+//                run under HMCECS in examples/covtype.py:154-165
+//
+// For the logistic likelihood the proxy of row i needs four numbers at the
+// reference point: l_ref = x_i.beta_ref, ll_ref = log Bern(y_i | l_ref),
+// r = y_i - sigmoid(l_ref) (gradient r x_i) and w = sigmoid(l_ref)(1 -
+// sigmoid(l_ref)) (Hessian -w x_i x_i^T), so the per-row proxy is
+// ll_ref + r dl - w dl^2 / 2 with dl = x_i.(beta - beta_ref): no per-row
+// D-vector or DxD matrix is stored.  The full-data proxy uses LL = sum ll_ref,
+// G = sum r x_i and H = -sum w x_i x_i^T, computed once at set-up.
+//
+// Layout: X row-major [n, d] fp32 (the caller's buffer), y [n] fp32, the
+// per-row scalars [n] fp64 each, H [d, d] fp64.  Subsample indices [C, m]
+// int32 and the HMCECS state keys [C, 2] are caller-owned device buffers.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <string>
+
+#include "nax_device.h"
+#include "nax_internal.h"
+
+namespace nax {
+
+constexpr int kEcsThreads = 256;
+constexpr int kEcsMaxD = 1024;
+constexpr int kEcsMaxM = 4096;
+constexpr int kEcsSplit = 256;  // row splits of the set-up reductions
+
+struct EcsArgs {
+  const float* x;
+  const float* y;
+  int64_t n;
+  int d, m, proxy;  // proxy: 0 none, 1 or 2 (Taylor degree)
+  double prior_scale;
+  const double* bref;
+  const double* l_ref;
+  const double* ll_ref;
+  const double* r;
+  const double* w;
+  const double* G;   // [d]
+  const double* H;   // [d, d]
+  double LL;
+};
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int wv = threadIdx.x / NAX_WAVE, ln = threadIdx.x % NAX_WAVE;
+  __syncthreads();
+  if (ln == 0) red[wv] = v;
+  __syncthreads();
+  double t = 0;
+  for (int i = 0; i < kEcsThreads / NAX_WAVE; ++i) t += red[i];
+  return t;
+}
+
+// log Bernoulli(y | logits l) in fp64: y l - softplus(l).
+__device__ __forceinline__ double bern_ll(double l, double y) {
+  return y * l - (fmax(l, 0.0) + log1p(exp(-fabs(l))));
+}
+
+// ---- set-up at the reference point ------------------------------------------
+
+// One wavefront per row: l_ref, ll_ref, r, w.
+__global__ __launch_bounds__(kEcsThreads) void ecs_rows_kernel(EcsArgs a, double* l_ref,
+                                                               double* ll_ref, double* r,
+                                                               double* w) {
+  const int ln = threadIdx.x % NAX_WAVE;
+  const int64_t waves = (int64_t)gridDim.x * (kEcsThreads / NAX_WAVE);
+  for (int64_t i = (int64_t)blockIdx.x * (kEcsThreads / NAX_WAVE) + threadIdx.x / NAX_WAVE;
+       i < a.n; i += waves) {
+    const float* xi = a.x + i * a.d;
+    double s = 0;
+    for (int k = ln; k < a.d; k += NAX_WAVE) s += (double)xi[k] * a.bref[k];
+    s = wave_sum(s);
+    if (ln == 0) {
+      const double y = a.y[i];
+      const double p = expit(s);
+      l_ref[i] = s;
+      ll_ref[i] = bern_ll(s, y);
+      r[i] = y - p;
+      w[i] = a.proxy == 2 ? p * (1.0 - p) : 0.0;
+    }
+  }
+}
+
+// Partial sums over row split blockIdx.y: part[s][k] = sum r_i x_ik (k < d),
+// part[s][d] = sum ll_ref_i.
+__global__ __launch_bounds__(kEcsThreads) void ecs_grad_sum_kernel(EcsArgs a, double* part) {
+  const int64_t rows = (a.n + kEcsSplit - 1) / kEcsSplit;
+  const int64_t i0 = blockIdx.y * rows, i1 = min(a.n, i0 + rows);
+  const int k = blockIdx.x * kEcsThreads + threadIdx.x;
+  if (k > a.d) return;
+  double s = 0;
+  if (k < a.d)
+    for (int64_t i = i0; i < i1; ++i) s += a.r[i] * (double)a.x[i * a.d + k];
+  else
+    for (int64_t i = i0; i < i1; ++i) s += a.ll_ref[i];
+  part[(int64_t)blockIdx.y * (a.d + 1) + k] = s;
+}
+
+// Partial Hessian over row split blockIdx.z for the 64x64 tile (blockIdx.x,
+// blockIdx.y): part[s][i][j] = -sum w_r x_ri x_rj.  Rows are staged 32 at a
+// time through LDS; each thread owns a 4x4 block of the tile.
+__global__ __launch_bounds__(kEcsThreads) void ecs_hess_kernel(EcsArgs a, double* part) {
+  __shared__ float xi[32][64];
+  __shared__ float xj[32][64];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int64_t rows = (a.n + kEcsSplit - 1) / kEcsSplit;
+  const int64_t r0 = blockIdx.z * rows, r1 = min(a.n, r0 + rows);
+  const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
+  double acc[4][4] = {};
+  for (int64_t rb = r0; rb < r1; rb += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 64; e += kEcsThreads) {
+      const int rr = e / 64, cc = e % 64;
+      const int64_t row = rb + rr;
+      const bool ok = row < r1;
+      const float wr = ok ? (float)a.w[row] : 0.0f;
+      xi[rr][cc] = (ok && i0 + cc < a.d) ? wr * a.x[row * a.d + i0 + cc] : 0.0f;
+      xj[rr][cc] = (ok && j0 + cc < a.d) ? a.x[row * a.d + j0 + cc] : 0.0f;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 32; ++rr) {
+      float p[4], q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) { p[u] = xi[rr][ti * 4 + u]; q[u] = xj[rr][tj * 4 + u]; }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += (double)(p[u] * q[v]);
+    }
+  }
+  double* out = part + (int64_t)blockIdx.z * a.d * a.d;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + ti * 4 + u, j = j0 + tj * 4 + v;
+      if (i < a.d && j < a.d) out[(int64_t)i * a.d + j] = -acc[u][v];
+    }
+}
+
+// out[k] = sum_s part[s][k], k < len.
+__global__ void ecs_reduce_kernel(const double* part, int nsplit, int64_t len, double* out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= len) return;
+  double s = 0;
+  for (int i = 0; i < nsplit; ++i) s += part[(int64_t)i * len + k];
+  out[k] = s;
+}
+
+// ---- the subsampled potential ------------------------------------------------
+
+// One workgroup per chain (act[blockIdx.x] when a list is given).  Pass 1:
+// each wavefront takes every 4th subsample row, lanes over the coordinates,
+// and keeps diff_j and d diff_j / d l_j in LDS.  Then the estimator's mean
+// and variance, and pass 2: thread k accumulates sum_j coef_j x_{i_j, k}
+// (coalesced row reads), plus (H delta)_k.
+__global__ __launch_bounds__(kEcsThreads) void ecs_eval_kernel(
+    EcsArgs a, const int* __restrict__ idx_all, const float* __restrict__ z, int ldz,
+    const int* act, const int* nact, double* pe, float* g, int ldg) {
+  __shared__ float beta[kEcsMaxD];
+  __shared__ double dlt[kEcsMaxD];
+  __shared__ double dif[kEcsMaxM];
+  __shared__ float coef[kEcsMaxM];
+  __shared__ double red[kEcsThreads / NAX_WAVE];
+  int c = blockIdx.x;
+  if (act) {
+    if (c >= *nact) return;
+    c = act[c];
+  }
+  const int d = a.d, m = a.m;
+  const float* zc = z + (int64_t)c * ldz;
+  const int* idx = idx_all + (int64_t)c * m;
+  for (int k = threadIdx.x; k < d; k += kEcsThreads) {
+    beta[k] = zc[k];
+    dlt[k] = a.proxy ? (double)zc[k] - a.bref[k] : 0.0;
+  }
+  __syncthreads();
+  const int wv = threadIdx.x / NAX_WAVE, ln = threadIdx.x % NAX_WAVE;
+  const double nm = (double)a.n / m;
+  double sdiff = 0;  // lane 0 of each wave
+  for (int j = wv; j < m; j += kEcsThreads / NAX_WAVE) {
+    const int64_t i = idx[j];
+    const float* xi = a.x + i * d;
+    float s = 0;
+    for (int k = ln; k < d; k += NAX_WAVE) s = fmaf(xi[k], beta[k], s);
+    s = wave_sum(s);
+    if (ln == 0) {
+      const double l = s, y = a.y[i];
+      const double ll = bern_ll(l, y);
+      const double dy = y - expit(l);
+      if (a.proxy) {
+        const double dl = l - a.l_ref[i];
+        const double df = ll - (a.ll_ref[i] + a.r[i] * dl - 0.5 * a.w[i] * dl * dl);
+        dif[j] = df;
+        coef[j] = (float)(dy - a.r[i] + a.w[i] * dl);
+        sdiff += df;
+      } else {
+        dif[j] = ll;
+        coef[j] = (float)(nm * dy);
+        sdiff += ll;
+      }
+    }
+  }
+  const double mean = block_sum(sdiff, red) / m;  // block_sum synchronises first
+  double est;
+  if (a.proxy) {
+    double sq = 0;
+    for (int j = threadIdx.x; j < m; j += kEcsThreads) {
+      const double t = dif[j] - mean;
+      sq += t * t;
+    }
+    const double var = block_sum(sq, red) / m;
+    // weight of diff_j: n/m (1 - n/m (diff_j - mean))
+    for (int j = threadIdx.x; j < m; j += kEcsThreads)
+      coef[j] = (float)(nm * (1.0 - nm * (dif[j] - mean)) * coef[j]);
+    // full-data proxy: LL + G.delta + delta.H delta / 2
+    double gd = 0;
+    for (int k = threadIdx.x; k < d; k += kEcsThreads) {
+      double hk = 0;
+      for (int l2 = 0; l2 < d; ++l2) hk += a.H[(int64_t)l2 * d + k] * dlt[l2];
+      gd += dlt[k] * (a.G[k] + 0.5 * hk);
+    }
+    const double full = a.LL + block_sum(gd, red);
+    est = full + (double)a.n * mean - 0.5 * (double)a.n * nm * var;
+  } else {
+    est = (double)a.n * mean;
+  }
+  __syncthreads();
+  // prior: sum log N(beta_k | 0, s)
+  const double s2 = a.prior_scale * a.prior_scale;
+  const double cst = log(a.prior_scale) + 0.91893853320467274178;  // log s + log(2 pi) / 2
+  double pr = 0;
+  for (int k = threadIdx.x; k < d; k += kEcsThreads)
+    pr += 0.5 * (double)beta[k] * beta[k] / s2 + cst;
+  pr = block_sum(pr, red);
+  if (threadIdx.x == 0) pe[c] = pr - est;
+  // gradient
+  for (int k = threadIdx.x; k < d; k += kEcsThreads) {
+    double acc = 0;
+    for (int j = 0; j < m; ++j) acc += (double)coef[j] * a.x[(int64_t)idx[j] * d + k];
+    if (a.proxy) {
+      double hk = 0;
+      for (int l2 = 0; l2 < d; ++l2) hk += a.H[(int64_t)l2 * d + k] * dlt[l2];
+      acc += a.G[k] + hk;
+    }
+    g[(int64_t)c * ldg + k] = (float)((double)beta[k] / s2 - acc);
+  }
+}
+
+// ---- the Gibbs update ---------------------------------------------------------
+
+// jax.random.randint(key, (n,), lo, hi)[i] for int32 (two 32-bit draws from
+// split(key), combined as (hi % span) * (2^32 mod span) + lo % span in
+// wrapping uint32, mod span).  n = 0 means the scalar shape.
+__device__ __forceinline__ int randint_at(Key key, uint32_t i, uint32_t n, bool part, int lo,
+                                          int hi) {
+  const Key k1 = split_key(key, 0, 2, part), k2 = split_key(key, 1, 2, part);
+  const uint32_t bh = n ? bits_vec(k1, i, n, part) : bits_scalar(k1, part);
+  const uint32_t bl = n ? bits_vec(k2, i, n, part) : bits_scalar(k2, part);
+  const uint32_t span = hi > lo ? (uint32_t)(hi - lo) : 1u;
+  uint32_t mult = 65536u % span;
+  mult = (mult * mult) % span;
+  const uint32_t off = ((bh % span) * mult + bl % span) % span;
+  return lo + (int)off;
+}
+
+// Per chain: advance the state key (split(key)[0]), pick a block with
+// split(key', 3)[2] and redraw its indices with split(key', 3)[1].
+__global__ __launch_bounds__(kEcsThreads) void ecs_propose_kernel(int C, int m, int nb, int64_t n,
+                                                                  bool part, uint32_t* keys,
+                                                                  const int* idx, int* prop) {
+  const int c = blockIdx.x;
+  if (c >= C) return;
+  const Key k0{keys[2 * c], keys[2 * c + 1]};
+  const Key rk = split_key(k0, 0, 2, part);
+  const Key sub = split_key(rk, 1, 3, part), blk = split_key(rk, 2, 3, part);
+  const int bs = (m - 1) / nb + 1;
+  const int pad = bs - (m - 1) % bs - 1;
+  int start = randint_at(blk, 0, 0, part, 0, nb) * bs;
+  start = min(start, m + pad - bs);
+  const int* ic = idx + (int64_t)c * m;
+  int* pc = prop + (int64_t)c * m;
+  for (int j = threadIdx.x; j < m; j += kEcsThreads) {
+    const int t = j - start;
+    pc[j] = (t >= 0 && t < bs) ? randint_at(sub, t, bs, part, 0, (int)n) : ic[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) { keys[2 * c] = rk.k0; keys[2 * c + 1] = rk.k1; }
+}
+
+// Per chain: accept the proposal with probability min(1, exp(pe - pe_new))
+// by bernoulli(key', p); on acceptance the indices, the potential energy and
+// the gradient (gst, the sampler's state gradient) take the proposal's values.
+__global__ __launch_bounds__(kEcsThreads) void ecs_accept_kernel(
+    int C, int m, int D, bool part, const uint32_t* keys, const double* pe_new, const float* g_new,
+    int ldg, const int* prop, int* idx, ChainState* st, float* gst, int ldv, float* accept_out) {
+  const int c = blockIdx.x;
+  if (c >= C) return;
+  const Key rk{keys[2 * c], keys[2 * c + 1]};
+  const double pe = st[c].pe;
+  const double p = fmin(exp(pe - pe_new[c]), 1.0);
+  const bool acc = bernoulli(rk, p, part);
+  if (acc) {
+    for (int j = threadIdx.x; j < m; j += kEcsThreads)
+      idx[(int64_t)c * m + j] = prop[(int64_t)c * m + j];
+    float* gv = gst + (size_t)c * ldv;
+    for (int k = threadIdx.x; k < D; k += kEcsThreads) gv[k] = g_new[(int64_t)c * ldg + k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (acc) st[c].pe = pe_new[c];
+    if (accept_out) accept_out[c] = (float)p;
+  }
+}
+
+__global__ void randint_kernel(const uint32_t* keys, int nk, int n, int lo, int hi, bool part,
+                               int* out) {
+  const int c = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nk || i >= n) return;
+  out[(int64_t)c * n + i] = randint_at(Key{keys[2 * c], keys[2 * c + 1]}, i, n, part, lo, hi);
+}
+
+__global__ void split_many_kernel(const uint32_t* keys, int nk, int num, bool part,
+                                  uint32_t* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nk * num) return;
+  const int c = t / num, j = t % num;
+  const Key k = split_key(Key{keys[2 * c], keys[2 * c + 1]}, j, num, part);
+  out[2 * t] = k.k0;
+  out[2 * t + 1] = k.k1;
+}
+
+struct EcsLogistic : Potential {
+  EcsArgs a{};
+  int C = 0, nb = 1, Dp = 0;
+  bool part = true;
+  int* idx = nullptr;     // caller-owned [C, m]
+  uint32_t* keys = nullptr;  // caller-owned [C, 2]
+  int* prop = nullptr;
+  double* pe_new = nullptr;
+  float* g_new = nullptr;
+  double* buf = nullptr;  // bref, per-row scalars, G, H
+  ~EcsLogistic() override {
+    hipFree(prop);
+    hipFree(pe_new);
+    hipFree(g_new);
+    hipFree(buf);
+  }
+  int dim() const override { return a.d; }
+  const char* name() const override { return "ecs_eval_kernel"; }
+  // per chain: each subsample row once (4d), its label and index, and the
+  // four fp64 proxy scalars
+  double alg_bytes(int C_) const override {
+    return (double)C_ * a.m * ((double)a.d * 4 + 4 + 4 + (a.proxy ? 32 : 0));
+  }
+  double alg_flops(int C_) const override { return 4.0 * C_ * a.m * a.d; }
+  hipError_t launch(hipStream_t s, const int* ix, const float* z, int ldz, int grid, const int* act,
+                    const int* nact, double* pe, float* g, int ldg, bool mark = true) {
+    if (mark) mark_begin(s);
+    hipLaunchKernelGGL(ecs_eval_kernel, dim3(grid), dim3(kEcsThreads), 0, s, a, ix, z, ldz, act,
+                       nact, pe, g, ldg);
+    if (mark) mark_end(s);
+    return hipGetLastError();
+  }
+  hipError_t eval(hipStream_t s, const float* z, int ldz, int C_, double* pe, float* g,
+                  int ldg) override {
+    return launch(s, idx, z, ldz, C_, nullptr, nullptr, pe, g, ldg);
+  }
+  hipError_t eval_active(hipStream_t s, const float* z, int ldz, int C_, const int* act,
+                         const int* nact, double* pe, float* g, int ldg) override {
+    return launch(s, idx, z, ldz, C_, act, nact, pe, g, ldg);
+  }
+  hipError_t ecs_gibbs(hipStream_t s, ChainState* st, const float* z, float* gst, int ldv,
+                       float* accept) override {
+    hipLaunchKernelGGL(ecs_propose_kernel, dim3(C), dim3(kEcsThreads), 0, s, C, a.m, nb, a.n,
+                       part, keys, idx, prop);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    // U and dU at each chain's current position with the proposed indices
+    err = launch(s, prop, z, ldv, C, nullptr, nullptr, pe_new, g_new, Dp, false);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(ecs_accept_kernel, dim3(C), dim3(kEcsThreads), 0, s, C, a.m, a.d, part,
+                       keys, pe_new, g_new, Dp, prop, idx, st, gst, ldv, accept);
+    return hipGetLastError();
+  }
+};
+
+Potential* make_ecs_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
+                             int m, int num_blocks, int proxy, const double* ref, int* idx,
+                             uint32_t* keys, int num_chains, int Dp, bool partitionable,
+                             hipStream_t s, std::string* err) {
+  if (n <= 0 || d <= 0 || d > kEcsMaxD) { *err = "ecs model: need n > 0 and 0 < dim <= 1024"; return nullptr; }
+  if (m <= 0 || m > kEcsMaxM || m > n) {
+    *err = "ecs model: subsample_size must be in [1, min(n, 4096)]";
+    return nullptr;
+  }
+  if (num_blocks <= 0) { *err = "ecs model: num_blocks must be positive"; return nullptr; }
+  if (proxy < 0 || proxy > 2) { *err = "ecs model: proxy degree must be 0 (none), 1 or 2"; return nullptr; }
+  if (proxy && !ref) { *err = "ecs model: a Taylor proxy needs reference coefficients"; return nullptr; }
+  if (!(prior_scale > 0)) { *err = "prior_scale must be positive"; return nullptr; }
+  if (!idx || !keys) { *err = "ecs model: null index or key buffer"; return nullptr; }
+  if (n > (int64_t)INT32_MAX) { *err = "ecs model: n must fit int32 indices"; return nullptr; }
+  auto* p = new EcsLogistic();
+  EcsArgs& a = p->a;
+  a.x = x; a.y = y; a.n = n; a.d = d; a.m = m; a.proxy = proxy; a.prior_scale = prior_scale;
+  p->C = num_chains; p->nb = num_blocks; p->Dp = Dp; p->part = partitionable;
+  p->idx = idx; p->keys = keys;
+  auto fail = [&](const char* what, hipError_t e) -> Potential* {
+    *err = std::string(what) + ": " + hipGetErrorString(e);
+    delete p;
+    return nullptr;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&p->prop, sizeof(int) * (size_t)num_chains * m))) return fail("hipMalloc", e);
+  if ((e = hipMalloc(&p->pe_new, sizeof(double) * num_chains))) return fail("hipMalloc", e);
+  if ((e = hipMalloc(&p->g_new, sizeof(float) * (size_t)num_chains * Dp))) return fail("hipMalloc", e);
+  if (!proxy) return p;
+  const size_t nbuf = (size_t)d + 4 * (size_t)n + d + (size_t)d * d;
+  if ((e = hipMalloc(&p->buf, sizeof(double) * nbuf))) return fail("hipMalloc", e);
+  double* bref = p->buf;
+  double* rows = bref + d;
+  a.bref = bref;
+  a.l_ref = rows; a.ll_ref = rows + n; a.r = rows + 2 * n; a.w = rows + 3 * n;
+  double* G = rows + 4 * n;
+  double* H = G + d;
+  a.G = G; a.H = H;
+  if ((e = hipMemcpyAsync(bref, ref, sizeof(double) * d, hipMemcpyDeviceToDevice, s)))
+    return fail("hipMemcpyAsync", e);
+  hipLaunchKernelGGL(ecs_rows_kernel, dim3(2048), dim3(kEcsThreads), 0, s, a,
+                     (double*)a.l_ref, (double*)a.ll_ref, (double*)a.r, (double*)a.w);
+  if ((e = hipGetLastError())) return fail("ecs_rows_kernel", e);
+  double* part = nullptr;
+  const size_t npart = (size_t)kEcsSplit * ((size_t)d * d > (size_t)d + 1 ? (size_t)d * d : d + 1);
+  if ((e = hipMallocAsync((void**)&part, sizeof(double) * npart, s))) return fail("hipMallocAsync", e);
+  hipLaunchKernelGGL(ecs_grad_sum_kernel, dim3((d + 1 + kEcsThreads - 1) / kEcsThreads, kEcsSplit),
+                     dim3(kEcsThreads), 0, s, a, part);
+  double* gl = nullptr;
+  if ((e = hipMallocAsync((void**)&gl, sizeof(double) * (d + 1), s))) return fail("hipMallocAsync", e);
+  hipLaunchKernelGGL(ecs_reduce_kernel, dim3((d + 1 + 255) / 256), dim3(256), 0, s, part,
+                     kEcsSplit, (int64_t)d + 1, gl);
+  double LL = 0;
+  if ((e = hipMemcpyAsync(G, gl, sizeof(double) * d, hipMemcpyDeviceToDevice, s))) return fail("copy", e);
+  if ((e = hipMemcpyAsync(&LL, gl + d, sizeof(double), hipMemcpyDeviceToHost, s))) return fail("copy", e);
+  if (proxy == 2) {
+    hipLaunchKernelGGL(ecs_hess_kernel, dim3((d + 63) / 64, (d + 63) / 64, kEcsSplit),
+                       dim3(kEcsThreads), 0, s, a, part);
+    hipLaunchKernelGGL(ecs_reduce_kernel, dim3((unsigned)(((int64_t)d * d + 255) / 256)), dim3(256),
+                       0, s, part, kEcsSplit, (int64_t)d * d, H);
+  } else {
+    if ((e = hipMemsetAsync(H, 0, sizeof(double) * d * d, s))) return fail("hipMemsetAsync", e);
+  }
+  if ((e = hipGetLastError())) return fail("ecs set-up", e);
+  hipFreeAsync(part, s);
+  hipFreeAsync(gl, s);
+  if ((e = hipStreamSynchronize(s))) return fail("ecs set-up", e);
+  a.LL = LL;
+  return p;
+}
+
+}  // namespace nax
+
+// random.randint(keys[c], (n,), lo, hi) for each of nk keys: out [nk, n].
+extern "C" int nax_randint(const uint32_t* keys, int32_t nk, int32_t n, int32_t lo, int32_t hi,
+                           int32_t partitionable, int32_t* out) {
+  if (!keys || !out || nk <= 0 || n <= 0) {
+    nax::set_error("nax_randint: null buffer or empty shape");
+    return NAX_EINVAL;
+  }
+  hipLaunchKernelGGL(nax::randint_kernel, dim3((n + 255) / 256, nk), dim3(256), 0, nullptr, keys,
+                     nk, n, lo, hi, partitionable != 0, out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    nax::set_error(std::string("nax_randint: ") + hipGetErrorString(e));
+    return NAX_EHIP;
+  }
+  return NAX_OK;
+}
+
+// random.split(keys[c], num) for each of nk keys: out [nk, num, 2].
+extern "C" int nax_split_keys(const uint32_t* keys, int32_t nk, int32_t num,
+                              int32_t partitionable, uint32_t* out) {
+  if (!keys || !out || nk <= 0 || num <= 0) {
+    nax::set_error("nax_split_keys: null buffer or empty shape");
+    return NAX_EINVAL;
+  }
+  const int64_t tot = (int64_t)nk * num;
+  hipLaunchKernelGGL(nax::split_many_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     nullptr, keys, nk, num, partitionable != 0, out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    nax::set_error(std::string("nax_split_keys: ") + hipGetErrorString(e));
+    return NAX_EHIP;
+  }
+  return NAX_OK;
+}

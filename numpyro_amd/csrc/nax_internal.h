@@ -106,6 +106,13 @@ struct Potential {
                                     float* zreq, float* gout, double* peout, int64_t* gevals) {
     return hipErrorNotSupported;
   }
+  // HMCECS models: one Gibbs update of every chain's subsample at its
+  // current position (z, stride ldv), accepting into st[c].pe and the state
+  // gradient gst (stride ldv); accept[c] receives the acceptance probability.
+  virtual hipError_t ecs_gibbs(hipStream_t s, ChainState* st, const float* z, float* gst, int ldv,
+                               float* accept) {
+    return hipErrorNotSupported;
+  }
   // When set, the potential records these events on its stream immediately
   // before and after its main (dominant) kernel.
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -125,6 +132,11 @@ Potential* make_hier(const float* x, const float* y, const int64_t* row_off, con
                      hipStream_t s, std::string* err);
 Potential* make_sv(const float* returns, int T, int noncentred, double sigma_rate, double nu_rate,
                    hipStream_t s, std::string* err);
+
+Potential* make_ecs_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
+                             int m, int num_blocks, int proxy, const double* ref, int* idx,
+                             uint32_t* keys, int num_chains, int Dp, bool partitionable,
+                             hipStream_t s, std::string* err);
 
 void set_error(const std::string& msg);
 

@@ -442,6 +442,39 @@ extern "C" int nax_model_hier(nax_sampler* h, const float* x, const float* y,
   return NAX_OK;
 }
 
+extern "C" int nax_model_logistic_ecs(nax_sampler* h, const float* x, const float* y, int64_t n,
+                                      int32_t d, double prior_scale, int32_t subsample_size,
+                                      int32_t num_blocks, int32_t proxy_degree, const double* ref,
+                                      int32_t* idx, uint32_t* keys) {
+  if (!h || !x || !y) { set_error("nax_model_logistic_ecs: null argument"); return NAX_EINVAL; }
+  if (d != h->cfg.D) { set_error("logistic model dim does not match the sampler dim"); return NAX_EINVAL; }
+  std::string err;
+  Potential* p = make_ecs_logistic(x, y, n, d, prior_scale, subsample_size, num_blocks,
+                                   proxy_degree, ref, idx, keys, h->cfg.C, h->cfg.Dp,
+                                   h->cfg.partitionable != 0, h->stream, &err);
+  if (!p) { set_error(err); return NAX_EINVAL; }
+  delete h->pot;
+  h->pot = p;
+  h->external = false;
+  h->cfg.chain_dim = nullptr;
+  return NAX_OK;
+}
+
+extern "C" int nax_ecs_gibbs(nax_sampler* h, float* accept_prob) {
+  if (!h) { set_error("null handle"); return NAX_EINVAL; }
+  if (!h->initialized) { set_error("nax_ecs_gibbs before nax_init"); return NAX_ESTATE; }
+  if (!h->pot) { set_error("nax_ecs_gibbs: no subsampled model set"); return NAX_ESTATE; }
+  const EngineCfg& e = h->cfg;
+  hipError_t r = h->pot->ecs_gibbs(h->stream, h->st, h->vec + (size_t)V_Z * e.Dp,
+                                   h->vec + (size_t)V_G * e.Dp, e.nvec * e.Dp, accept_prob);
+  if (r == hipErrorNotSupported) {
+    set_error("nax_ecs_gibbs: the model has no subsample plate (use nax_model_logistic_ecs)");
+    return NAX_ESTATE;
+  }
+  NAX_HIP(r);
+  return NAX_OK;
+}
+
 extern "C" int nax_model_external(nax_sampler* h) {
   if (!h) { set_error("null handle"); return NAX_EINVAL; }
   delete h->pot;

@@ -98,6 +98,10 @@ class Engine:
         model.attach(h)
         self.external = isinstance(model, PotentialFn)
         self.grad_evals = 0
+        # HMCECS: a Gibbs update of the subsample before every iteration
+        self.gibbs = hasattr(model, "num_blocks")
+        self.gibbs_accept = (torch.zeros(self.C, dtype=torch.float32, device="cuda")
+                             if self.gibbs else None)
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -172,6 +176,17 @@ class Engine:
         torch.cuda.synchronize()
         if self.external:
             self.grad_evals = self._advance_external(target, C.byref(col))
+        elif self.gibbs:
+            # HMCECS.sample (numpyro/infer/hmc_gibbs.py:640-681): Gibbs update
+            # of the subsample, then one iteration of the inner kernel
+            total = 0
+            for t in range(int(self.get_state().i.max()), int(target)):
+                _lib.check(self.lib.nax_ecs_gibbs(self.h, _lib.ptr(self.gibbs_accept)),
+                           "nax_ecs_gibbs")
+                ge = C.c_int64()
+                _lib.check(self.lib.nax_run(self.h, t + 1, C.byref(col), C.byref(ge)), "nax_run")
+                total += ge.value
+            self.grad_evals = total
         else:
             ge = C.c_int64()
             _lib.check(self.lib.nax_run(self.h, int(target), C.byref(col), C.byref(ge)),
@@ -412,6 +427,11 @@ class HMC(MCMCKernel):
             # NUTS(model): trace the model once and bind the matching kernels
             # (the reference's initialize_model, numpyro/infer/util.py:663-835)
             target = compile_model(self._model, *model_args, **(model_kwargs or {}))
+            if target.params.get("subsample"):
+                # numpyro/primitives.py:448-456
+                raise ValueError(
+                    "Missing random key to generate subsample indices. Algorithms like HMC/NUTS "
+                    "do not support subsampling. You might want to use SVI or HMCECS instead.")
             target.bind(num_chains)
             self._deterministic_sites = tuple(target.deterministic)
             if isinstance(init_params, dict):

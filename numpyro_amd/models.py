@@ -108,6 +108,64 @@ class LogisticRegression(Model):
         return elt * self.N * self.dim + 4.0 * self.N
 
 
+class SubsampledLogisticRegression(Model):
+    """The logistic model with a subsample plate over its rows
+    (``examples/covtype.py:66-71`` with ``subsample_size``), for ``HMCECS``
+    (``numpyro/infer/hmc_gibbs.py:502-``).  Each chain evaluates its own
+    subsample of ``subsample_size`` rows of X (fp32), with the plate's n/m
+    scaling (``proxy_degree=0``) or the difference estimator around
+    ``reference`` coefficients (``proxy_degree`` 1 or 2,
+    ``numpyro/contrib/ecs_proxies.py:23-50, 95-250``).  ``idx`` [C, m] int32
+    and ``keys`` [C, 2] (the HMCECS state keys, as int32 bit patterns) are
+    device tensors set by ``HMCECS`` before the engine is created."""
+
+    def __init__(self, X, y, subsample_size, prior_scale=1.0, num_blocks=1, proxy_degree=0,
+                 reference=None, site_sizes=None):
+        X = torch.as_tensor(X)
+        y = torch.as_tensor(y)
+        if X.ndim != 2 or y.ndim != 1 or y.shape[0] != X.shape[0]:
+            raise ValueError("X must be [N, D] and y [N]")
+        self.X = X.to(device="cuda", dtype=torch.float32).contiguous()
+        self.y = y.to(device="cuda", dtype=torch.float32).contiguous()
+        self.N, self.dim = self.X.shape
+        self.valid_obs = _check_binary_obs(self.y)
+        self.prior_scale = float(prior_scale)
+        self.m = int(subsample_size)
+        if not 0 < self.m <= min(self.N, 4096):
+            raise ValueError("subsample_size must be in [1, min(N, 4096)]")
+        self.num_blocks = int(num_blocks)
+        if self.num_blocks < 1:
+            raise ValueError("num_blocks must be positive")
+        self.proxy_degree = int(proxy_degree)
+        if self.proxy_degree not in (0, 1, 2):
+            raise ValueError("Taylor proxy only defined for first and second degree.")
+        self.reference = None
+        if self.proxy_degree:
+            ref = torch.as_tensor(reference, dtype=torch.float64).reshape(-1)
+            if ref.shape[0] != self.dim:
+                raise ValueError(f"reference coefficients must have {self.dim} entries")
+            self.reference = ref.to("cuda").contiguous()
+        self.site_sizes = tuple(site_sizes) if site_sizes else (("coefs", self.dim, None),)
+        self.idx = None
+        self.keys = None
+
+    def attach(self, handle):
+        if self.idx is None or self.keys is None:
+            raise RuntimeError("subsampled model: set idx and keys (HMCECS.init) first")
+        _lib.check(_lib.load().nax_model_logistic_ecs(
+            handle, _lib.ptr(self.X), _lib.ptr(self.y), self.N, self.dim, self.prior_scale,
+            self.m, self.num_blocks, self.proxy_degree, _lib.ptr(self.reference),
+            _lib.ptr(self.idx), _lib.ptr(self.keys)), "subsampled logistic model")
+
+    def algorithmic_flops(self, num_chains):
+        return 4.0 * self.m * self.dim * num_chains
+
+    def algorithmic_bytes(self, num_chains):
+        # per chain: each subsample row once, its label and index, and the
+        # four fp64 proxy scalars of the row
+        return num_chains * self.m * (4.0 * self.dim + 8.0 + (32.0 if self.proxy_degree else 0))
+
+
 def _canon(x_dtype):
     return {0: "fp32", 1: "bf16", 2: "fp8"}[_lib.X_DTYPES[x_dtype]]
 

@@ -86,12 +86,17 @@ def deterministic(name, value):
 
 
 class plate(Messenger):
+    """A plate of ``size``; with ``subsample_size < size`` it is a subsample
+    plate for HMCECS.  The model compiler traces such a plate over all of its
+    rows (the returned indices are ``arange(size)``): the HMCECS kernels draw
+    and update each chain's subsample on the GPU."""
+
     def __init__(self, name, size, subsample_size=None, dim=None):
         self.name = name
         self.size = int(size)
-        if subsample_size is not None and int(subsample_size) != self.size:
-            raise NotImplementedError(
-                "subsampling plates (HMCECS) are not supported: subsample_size must be None")
+        self.subsample_size = self.size if subsample_size is None else int(subsample_size)
+        if not 0 < self.subsample_size <= self.size:
+            raise ValueError("subsample_size must be in [1, size]")
         if dim is not None and dim >= 0:
             raise ValueError("dim arg must be negative.")
         self.dim = dim
@@ -106,6 +111,18 @@ class plate(Messenger):
             self.dim = d
         super().__enter__()
         return torch.arange(self.size)
+
+
+def subsample(data, event_dim):
+    """``primitives.py:640-680``: ``data`` indexed by the enclosing subsample
+    plates' indices, which are every row under the model compiler."""
+    for p in _STACK:
+        if isinstance(p, plate) and p.subsample_size < p.size:
+            dim = p.dim - event_dim
+            if data.shape[dim] != p.size:
+                raise ValueError(f"subsample: data has {data.shape[dim]} rows at dim {dim}, "
+                                 f"plate {p.name!r} has {p.size}")
+    return data
 
 
 class trace(Messenger):

@@ -184,7 +184,7 @@ def test_unsupported_models_are_refused(which):
             npa.sample("y", dist.Normal(X @ b, 1.0), obs=y)
         elif which == "subsample":
             with npa.plate("N", 50, subsample_size=10):
-                npa.sample("y", dist.Bernoulli(logits=X @ b), obs=y)
+                npa.sample("y", dist.Normal(X @ b, 1.0), obs=y)
         elif which == "probs":
             npa.sample("y", dist.Bernoulli(probs=X @ b), obs=y)
         elif which == "two_x":
@@ -253,3 +253,33 @@ def test_intercept_with_other_scale_is_refused():
 
     with pytest.raises(NotImplementedError):
         match_model(model)
+
+
+def _covtype_subsample_model(data, labels, subsample_size=None):
+    # examples/covtype.py:66-71
+    dim = data.shape[1]
+    coefs = npa.sample("coefs", dist.Normal(torch.zeros(dim), torch.ones(dim)))
+    with npa.plate("N", data.shape[0], subsample_size=subsample_size) as idx:
+        logits = torch.matmul(data[idx], coefs)
+        return npa.sample("obs", dist.Bernoulli(logits=logits), obs=labels[idx])
+
+
+def test_subsample_plate_is_recorded_for_hmcecs():
+    X, y = _logistic_data(200, 4)
+    kind, p, _ = match_model(_covtype_subsample_model, X, y, 25)
+    assert kind == "logistic" and p["subsample"] == ("N", 200, 25)
+    assert torch.equal(p["X"], X)
+    kind, p, _ = match_model(_covtype_subsample_model, X, y)
+    assert "subsample" not in p
+
+    def with_subsample_primitive(data, labels):
+        b = npa.sample("b", dist.Normal(0.0, 2.0).expand([4]).to_event(1))
+        with npa.plate("rows", data.shape[0], subsample_size=10):
+            xb = npa.subsample(data, event_dim=1)
+            yb = npa.subsample(labels, event_dim=0)
+            npa.sample("y", dist.Bernoulli(logits=xb @ b), obs=yb)
+
+    kind, p, _ = match_model(with_subsample_primitive, X, y)
+    assert p["subsample"] == ("rows", 200, 10) and p["prior_scale"] == 2.0
+    with pytest.raises(ValueError):
+        npa.plate("N", 10, subsample_size=11)
