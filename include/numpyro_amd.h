@@ -173,7 +173,7 @@ int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, in
  * difference estimator with a Taylor proxy around d_ref [d] fp64
  * (numpyro/contrib/ecs_proxies.py:23-50, 95-250).  d_keys [C, 2] are the
  * HMCECS state keys (HMCECSState.rng_key), advanced by nax_ecs_gibbs.
- * subsample_size <= 4096, dim <= 1024. */
+ * subsample_size <= 4096, dim <= 512. */
 int nax_model_logistic_ecs(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                            int32_t d, double prior_scale, int32_t subsample_size,
                            int32_t num_blocks, int32_t proxy_degree, const double* d_ref,

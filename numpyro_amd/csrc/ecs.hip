@@ -32,7 +32,7 @@
 namespace nax {
 
 constexpr int kEcsThreads = 256;
-constexpr int kEcsMaxD = 1024;
+constexpr int kEcsMaxD = 512;
 constexpr int kEcsMaxM = 4096;
 constexpr int kEcsSplit = 256;  // row splits of the set-up reductions
 
@@ -161,19 +161,25 @@ __global__ void ecs_reduce_kernel(const double* part, int nsplit, int64_t len, d
 
 // ---- the subsampled potential ------------------------------------------------
 
-// One workgroup per chain (act[blockIdx.x] when a list is given).  Pass 1:
-// each wavefront takes every 4th subsample row, lanes over the coordinates,
-// and keeps diff_j and d diff_j / d l_j in LDS.  Then the estimator's mean
-// and variance, and pass 2: thread k accumulates sum_j coef_j x_{i_j, k}
-// (coalesced row reads), plus (H delta)_k.
-__global__ __launch_bounds__(kEcsThreads) void ecs_eval_kernel(
+// One workgroup per chain (act[blockIdx.x] when a list is given), one pass
+// over the subsample.  Each wavefront takes ROWS rows at a time, lanes
+// over the coordinates (the row values stay in registers), reduces l = x.beta
+// for each row, and lane r forms the fp64 per-row terms of row r: diff_r and
+// c_r = d diff_r / d l_r.  The estimator's weight on diff_j,
+// n/m (1 - n/m (diff_j - mean)), is n/m (1 + n/m mean) - (n/m)^2 diff_j, so
+// the gradient needs only S1 = sum c_j x_j and S2 = sum diff_j c_j x_j, both
+// accumulated from the registers while the mean is still unknown.
+// NCH: 64-column chunks of a row held in registers (d <= 64 NCH); ROWS rows
+// per wavefront step (fewer for wide rows, to stay within 128 VGPRs).
+template <int NCH, int ROWS>
+__global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel(
     EcsArgs a, const int* __restrict__ idx_all, const float* __restrict__ z, int ldz,
     const int* act, const int* nact, double* pe, float* g, int ldg) {
+  constexpr int NW = kEcsThreads / NAX_WAVE;
   __shared__ float beta[kEcsMaxD];
   __shared__ double dlt[kEcsMaxD];
-  __shared__ double dif[kEcsMaxM];
-  __shared__ float coef[kEcsMaxM];
-  __shared__ double red[kEcsThreads / NAX_WAVE];
+  __shared__ double part[2][NW][kEcsMaxD];
+  __shared__ double red[NW];
   int c = blockIdx.x;
   if (act) {
     if (c >= *nact) return;
@@ -182,80 +188,134 @@ __global__ __launch_bounds__(kEcsThreads) void ecs_eval_kernel(
   const int d = a.d, m = a.m;
   const float* zc = z + (int64_t)c * ldz;
   const int* idx = idx_all + (int64_t)c * m;
-  for (int k = threadIdx.x; k < d; k += kEcsThreads) {
-    beta[k] = zc[k];
-    dlt[k] = a.proxy ? (double)zc[k] - a.bref[k] : 0.0;
+  for (int k = threadIdx.x; k < kEcsMaxD; k += kEcsThreads) {
+    beta[k] = k < d ? zc[k] : 0.0f;
+    dlt[k] = (a.proxy && k < d) ? (double)zc[k] - a.bref[k] : 0.0;
   }
   __syncthreads();
   const int wv = threadIdx.x / NAX_WAVE, ln = threadIdx.x % NAX_WAVE;
+  const int nch = (d + NAX_WAVE - 1) / NAX_WAVE;
   const double nm = (double)a.n / m;
-  double sdiff = 0;  // lane 0 of each wave
-  for (int j = wv; j < m; j += kEcsThreads / NAX_WAVE) {
-    const int64_t i = idx[j];
-    const float* xi = a.x + i * d;
-    float s = 0;
-    for (int k = ln; k < d; k += NAX_WAVE) s = fmaf(xi[k], beta[k], s);
-    s = wave_sum(s);
-    if (ln == 0) {
-      const double l = s, y = a.y[i];
+  double s1[NCH], s2[NCH];
+#pragma unroll
+  for (int t = 0; t < NCH; ++t) s1[t] = s2[t] = 0.0;
+  double sd = 0, sd2 = 0;  // lanes 0..ROWS-1: sum diff, sum diff^2
+  for (int j0 = wv * ROWS; j0 < m; j0 += NW * ROWS) {
+    const float* xr[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) xr[r] = a.x + (int64_t)idx[min(j0 + r, m - 1)] * d;
+    float xv[ROWS][NCH];
+    float s[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) s[r] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NCH; ++t) {
+      const int k = ln + t * NAX_WAVE;
+      const bool ok = t < nch && k < d;
+      const float b = beta[ok ? k : 0];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        xv[r][t] = ok ? xr[r][k] : 0.0f;
+        s[r] = fmaf(xv[r][t], b, s[r]);
+      }
+    }
+    float mine = 0.0f;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const float t = wave_sum(s[r]);
+      if (ln == r) mine = t;
+    }
+    const int j = j0 + ln;
+    double df = 0, cc = 0;
+    if (ln < ROWS && j < m) {
+      const int64_t i = idx[j];
+      const double l = mine, y = a.y[i];
       const double ll = bern_ll(l, y);
       const double dy = y - expit(l);
       if (a.proxy) {
         const double dl = l - a.l_ref[i];
-        const double df = ll - (a.ll_ref[i] + a.r[i] * dl - 0.5 * a.w[i] * dl * dl);
-        dif[j] = df;
-        coef[j] = (float)(dy - a.r[i] + a.w[i] * dl);
-        sdiff += df;
+        df = ll - (a.ll_ref[i] + a.r[i] * dl - 0.5 * a.w[i] * dl * dl);
+        cc = dy - a.r[i] + a.w[i] * dl;
       } else {
-        dif[j] = ll;
-        coef[j] = (float)(nm * dy);
-        sdiff += ll;
+        df = ll;
+        cc = nm * dy;
+      }
+      sd += df;
+      sd2 += df * df;
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const double cr = __shfl(cc, r, NAX_WAVE);
+      const double dcr = __shfl(df * cc, r, NAX_WAVE);
+#pragma unroll
+      for (int t = 0; t < NCH; ++t) {
+        if (t < nch) {
+          s1[t] += cr * xv[r][t];
+          s2[t] += dcr * xv[r][t];
+        }
       }
     }
   }
-  const double mean = block_sum(sdiff, red) / m;  // block_sum synchronises first
-  double est;
-  if (a.proxy) {
-    double sq = 0;
-    for (int j = threadIdx.x; j < m; j += kEcsThreads) {
-      const double t = dif[j] - mean;
-      sq += t * t;
+#pragma unroll
+  for (int t = 0; t < NCH; ++t) {
+    const int k = ln + t * NAX_WAVE;
+    if (t < nch && k < d) {
+      part[0][wv][k] = s1[t];
+      part[1][wv][k] = s2[t];
     }
-    const double var = block_sum(sq, red) / m;
-    // weight of diff_j: n/m (1 - n/m (diff_j - mean))
-    for (int j = threadIdx.x; j < m; j += kEcsThreads)
-      coef[j] = (float)(nm * (1.0 - nm * (dif[j] - mean)) * coef[j]);
+  }
+  // (H delta)_k for this thread's coordinates k = tid + 256 q, kept for the
+  // gradient; H is symmetric, so column reads are coalesced
+  double hk[kEcsMaxD / kEcsThreads];
+#pragma unroll
+  for (int q = 0; q < kEcsMaxD / kEcsThreads; ++q) {
+    hk[q] = 0.0;
+    const int k = threadIdx.x + q * kEcsThreads;
+    if (a.proxy && k < d)
+      for (int l2 = 0; l2 < d; ++l2) hk[q] += a.H[(int64_t)l2 * d + k] * dlt[l2];
+  }
+  const double mean = block_sum(sd, red) / m;  // block_sum synchronises first
+  double est, w1, w2;
+  if (a.proxy) {
+    const double var = fmax(block_sum(sd2, red) / m - mean * mean, 0.0);
     // full-data proxy: LL + G.delta + delta.H delta / 2
     double gd = 0;
-    for (int k = threadIdx.x; k < d; k += kEcsThreads) {
-      double hk = 0;
-      for (int l2 = 0; l2 < d; ++l2) hk += a.H[(int64_t)l2 * d + k] * dlt[l2];
-      gd += dlt[k] * (a.G[k] + 0.5 * hk);
+#pragma unroll
+    for (int q = 0; q < kEcsMaxD / kEcsThreads; ++q) {
+      const int k = threadIdx.x + q * kEcsThreads;
+      if (k < d) gd += dlt[k] * (a.G[k] + 0.5 * hk[q]);
     }
     const double full = a.LL + block_sum(gd, red);
     est = full + (double)a.n * mean - 0.5 * (double)a.n * nm * var;
+    w1 = nm * (1.0 + nm * mean);
+    w2 = nm * nm;
   } else {
     est = (double)a.n * mean;
+    w1 = 1.0;
+    w2 = 0.0;
   }
-  __syncthreads();
   // prior: sum log N(beta_k | 0, s)
-  const double s2 = a.prior_scale * a.prior_scale;
+  const double s2v = a.prior_scale * a.prior_scale;
   const double cst = log(a.prior_scale) + 0.91893853320467274178;  // log s + log(2 pi) / 2
   double pr = 0;
   for (int k = threadIdx.x; k < d; k += kEcsThreads)
-    pr += 0.5 * (double)beta[k] * beta[k] / s2 + cst;
-  pr = block_sum(pr, red);
+    pr += 0.5 * (double)beta[k] * beta[k] / s2v + cst;
+  pr = block_sum(pr, red);  // also orders the part[] writes before the reads below
   if (threadIdx.x == 0) pe[c] = pr - est;
-  // gradient
-  for (int k = threadIdx.x; k < d; k += kEcsThreads) {
-    double acc = 0;
-    for (int j = 0; j < m; ++j) acc += (double)coef[j] * a.x[(int64_t)idx[j] * d + k];
-    if (a.proxy) {
-      double hk = 0;
-      for (int l2 = 0; l2 < d; ++l2) hk += a.H[(int64_t)l2 * d + k] * dlt[l2];
-      acc += a.G[k] + hk;
+#pragma unroll
+  for (int q = 0; q < kEcsMaxD / kEcsThreads; ++q) {
+    const int k = threadIdx.x + q * kEcsThreads;
+    if (k < d) {
+      double t1 = 0, t2 = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        t1 += part[0][w][k];
+        t2 += part[1][w][k];
+      }
+      double acc = w1 * t1 - w2 * t2;
+      if (a.proxy) acc += a.G[k] + hk[q];
+      g[(int64_t)c * ldg + k] = (float)((double)beta[k] / s2v - acc);
     }
-    g[(int64_t)c * ldg + k] = (float)((double)beta[k] / s2 - acc);
   }
 }
 
@@ -361,6 +421,7 @@ struct EcsLogistic : Potential {
   }
   int dim() const override { return a.d; }
   const char* name() const override { return "ecs_eval_kernel"; }
+  static_assert(kEcsMaxD == 8 * NAX_WAVE, "the widest instantiation holds 8 column chunks");
   // per chain: each subsample row once (4d), its label and index, and the
   // four fp64 proxy scalars
   double alg_bytes(int C_) const override {
@@ -370,8 +431,15 @@ struct EcsLogistic : Potential {
   hipError_t launch(hipStream_t s, const int* ix, const float* z, int ldz, int grid, const int* act,
                     const int* nact, double* pe, float* g, int ldg, bool mark = true) {
     if (mark) mark_begin(s);
-    hipLaunchKernelGGL(ecs_eval_kernel, dim3(grid), dim3(kEcsThreads), 0, s, a, ix, z, ldz, act,
-                       nact, pe, g, ldg);
+    const dim3 gr(grid), bl(kEcsThreads);
+    if (a.d <= 64)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ecs_eval_kernel<1, 8>), gr, bl, 0, s, a, ix, z, ldz, act, nact, pe, g, ldg);
+    else if (a.d <= 128)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ecs_eval_kernel<2, 8>), gr, bl, 0, s, a, ix, z, ldz, act, nact, pe, g, ldg);
+    else if (a.d <= 256)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ecs_eval_kernel<4, 4>), gr, bl, 0, s, a, ix, z, ldz, act, nact, pe, g, ldg);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ecs_eval_kernel<8, 2>), gr, bl, 0, s, a, ix, z, ldz, act, nact, pe, g, ldg);
     if (mark) mark_end(s);
     return hipGetLastError();
   }
@@ -402,7 +470,7 @@ Potential* make_ecs_logistic(const float* x, const float* y, int64_t n, int d, d
                              int m, int num_blocks, int proxy, const double* ref, int* idx,
                              uint32_t* keys, int num_chains, int Dp, bool partitionable,
                              hipStream_t s, std::string* err) {
-  if (n <= 0 || d <= 0 || d > kEcsMaxD) { *err = "ecs model: need n > 0 and 0 < dim <= 1024"; return nullptr; }
+  if (n <= 0 || d <= 0 || d > kEcsMaxD) { *err = "ecs model: need n > 0 and 0 < dim <= 512"; return nullptr; }
   if (m <= 0 || m > kEcsMaxM || m > n) {
     *err = "ecs model: subsample_size must be in [1, min(n, 4096)]";
     return nullptr;
