@@ -173,6 +173,11 @@ class Engine:
                     setattr(col, _COLLECT_PTR[f], out[f].data_ptr())
                 else:
                     raise ValueError(f"cannot collect field {f!r}")
+        gib = None
+        if self.gibbs and size > 0 and "z" in collect_fields:
+            # the subsample of each collected iteration (HMCECS's Gibbs site)
+            gib = torch.zeros((self.C, size, self.model.m), dtype=torch.int32, device="cuda")
+            out["_gibbs"] = gib
         torch.cuda.synchronize()
         if self.external:
             self.grad_evals = self._advance_external(target, C.byref(col))
@@ -186,6 +191,11 @@ class Engine:
                 ge = C.c_int64()
                 _lib.check(self.lib.nax_run(self.h, t + 1, C.byref(col), C.byref(ge)), "nax_run")
                 total += ge.value
+                rel = t - int(start_idx)
+                if gib is not None and rel >= 0 and rel % thinning == thinning - 1 \
+                        and rel // thinning < size:
+                    gib[:, rel // thinning].copy_(self.model.idx)
+                    torch.cuda.synchronize()  # before the next update rewrites idx
             self.grad_evals = total
         else:
             ge = C.c_int64()

@@ -157,6 +157,10 @@ class MCMC:
         for f in collect_fields:
             if f == self._sample_field:
                 z = pp(out["z"])
+                gsites = getattr(self.sampler, "_gibbs_sites", ())
+                if "_gibbs" in out and isinstance(z, dict) and gsites:
+                    # HMCECS: z = {**z_gibbs, **hmc_state.z} (hmc_gibbs.py:680)
+                    z[gsites[0]] = out["_gibbs"]
                 if isinstance(z, dict):
                     for s in remove_sites:
                         z.pop(s, None)
@@ -193,6 +197,7 @@ class MCMC:
         elif isinstance(sites, dict):
             skip = set(getattr(self.sampler, "_deterministic_sites", ())) \
                 if exclude_deterministic else set()
+            skip |= set(getattr(self.sampler, "_gibbs_sites", ()))  # subsample indices
             sites = {k: v.cpu().numpy() for k, v in sites.items() if k not in skip}
         print_summary(sites, prob=prob)
         ex = self.get_extra_fields()

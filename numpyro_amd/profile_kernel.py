@@ -3,6 +3,7 @@
     python -m numpyro_amd.profile_kernel --config c5 --x-dtype bf16 --reps 20
     python -m numpyro_amd.profile_kernel --copy-mib 1024     # copy-kernel calibration
     python -m numpyro_amd.profile_kernel --config c4         # ragged hierarchical models
+    python -m numpyro_amd.profile_kernel --config ecs        # HMCECS subsampled potential
 
 Each rep evaluates U and dU for every chain at fixed positions (one launch
 of the main kernel plus the fixed-order reduction), timed with HIP events
@@ -20,7 +21,7 @@ import torch
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c4", "c3", "c3nc"])
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c4", "c3", "c3nc", "ecs"])
     ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "mfma", "single", "two-copy"])
     ap.add_argument("--reps", type=int, default=10)
@@ -54,6 +55,24 @@ def main(argv=None):
             gen = torch.Generator(device="cuda")
             gen.manual_seed(0)
             z = 0.5 * torch.randn((C_, D), generator=gen, device="cuda")
+            args.x_dtype = "fp32"
+        elif args.config == "ecs":
+            # the N=2M, D=256 design, 1024 chains, each on its own subsample of
+            # 1000 rows, Taylor proxy of degree 2 around the generating beta
+            import numpy as np
+            from . import random as nrandom
+            from .models import SubsampledLogisticRegression
+            X, y, beta, cfg = workloads.make("c5", N=args.rows, chains=args.chains)
+            C_, D = cfg["chains"], cfg["D"]
+            model = SubsampledLogisticRegression(X, y, 1000, proxy_degree=2, reference=beta)
+            keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C_)
+            kd = torch.as_tensor(np.asarray(keys, np.uint32).view(np.int32)).cuda()
+            model.keys = kd.contiguous()
+            model.idx = torch.empty((C_, 1000), dtype=torch.int32, device="cuda")
+            _lib.check(_lib.load().nax_randint(_lib.ptr(model.keys), C_, 1000, 0, X.shape[0],
+                                               int(nrandom.PARTITIONABLE), _lib.ptr(model.idx)),
+                       "randint")
+            z = (beta[None, :] + 1e-3 * torch.randn((C_, D), device="cuda")).contiguous()
             args.x_dtype = "fp32"
         else:
             X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)

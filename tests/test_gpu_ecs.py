@@ -166,6 +166,31 @@ def test_hmcecs_covtype_posterior(proxy):
     assert acc.shape == (4,) and np.all((acc >= 0) & (acc <= 1))
     idx = kern.subsample_indices["N"]
     assert idx.shape == (4, 1000) and int(idx.min()) >= 0 and int(idx.max()) < 20000
+    # the Gibbs site is collected with the samples (z = {**z_gibbs, **hmc_state.z})
+    drawn = mcmc.get_samples(group_by_chain=True)["N"]
+    assert drawn.shape == (4, 300, 1000) and drawn.dtype == torch.int32
+    assert torch.equal(drawn[:, -1], idx)
+    assert bool((drawn[:, 1:] != drawn[:, :-1]).any())
+    mcmc.print_summary()  # the integer Gibbs site is left out of the summary
+
+
+def test_hmcecs_with_hmc_and_first_order_proxy():
+    # test/infer/test_hmc_gibbs.py:302-330 runs HMCECS over HMC and NUTS with
+    # Taylor proxies of degree 1 and 2
+    from numpyro_amd import random
+    from numpyro_amd.infer import HMC, HMCECS, MCMC, NUTS
+    X, y, beta = _data(20000, 4, 9)
+    Xt, yt = torch.as_tensor(X), torch.as_tensor(y)
+    full = MCMC(NUTS(_covtype), num_warmup=200, num_samples=200, num_chains=2)
+    full.run(random.PRNGKey(3), Xt, yt)
+    ref = full.get_samples()["coefs"]
+    kern = HMCECS(HMC(_covtype, trajectory_length=1.0), num_blocks=20,
+                  proxy=HMCECS.taylor_proxy({"coefs": ref.mean(0)}, degree=1))
+    mcmc = MCMC(kern, num_warmup=200, num_samples=200, num_chains=2)
+    mcmc.run(random.PRNGKey(4), Xt, yt, 2000)
+    s = mcmc.get_samples()["coefs"]
+    sd = ref.std(0)
+    assert bool(((s.mean(0) - ref.mean(0)).abs() < 4 * sd).all()), (s.mean(0), ref.mean(0), sd)
 
 
 def test_plain_nuts_refuses_a_subsample_plate():
