@@ -19,7 +19,8 @@
 // G = sum r x_i and H = -sum w x_i x_i^T, computed once at set-up.
 //
 // Layout: X row-major [n, d] fp32 (the caller's buffer), y [n] fp32, the
-// per-row scalars [n] fp64 each, H [d, d] fp64.  Subsample indices [C, m]
+// per-row scalars as one 32-byte record per row (a gathered row costs one
+// 32-byte read, not four cache lines), H [d, d] fp64.  Subsample indices [C, m]
 // int32 and the HMCECS state keys [C, 2] are caller-owned device buffers.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -36,6 +37,15 @@ constexpr int kEcsMaxD = 512;
 constexpr int kEcsMaxM = 4096;
 constexpr int kEcsSplit = 256;  // row splits of the set-up reductions
 
+// Per-row Taylor terms.  y (0 or 1) is carried in the sign bit of w (w >= 0),
+// so the proxy's gathered reads of a row take one 32-byte record.
+struct alignas(32) EcsRow {
+  double l_ref, ll_ref, r, w_y;
+};
+
+__device__ __forceinline__ double ecs_w(const EcsRow& q) { return fabs(q.w_y); }
+__device__ __forceinline__ double ecs_y(const EcsRow& q) { return signbit(q.w_y) ? 1.0 : 0.0; }
+
 struct EcsArgs {
   const float* x;
   const float* y;
@@ -43,10 +53,7 @@ struct EcsArgs {
   int d, m, proxy;  // proxy: 0 none, 1 or 2 (Taylor degree)
   double prior_scale;
   const double* bref;
-  const double* l_ref;
-  const double* ll_ref;
-  const double* r;
-  const double* w;
+  const EcsRow* rows;  // [n]
   const double* G;   // [d]
   const double* H;   // [d, d]
   double LL;
@@ -70,10 +77,8 @@ __device__ __forceinline__ double bern_ll(double l, double y) {
 
 // ---- set-up at the reference point ------------------------------------------
 
-// One wavefront per row: l_ref, ll_ref, r, w.
-__global__ __launch_bounds__(kEcsThreads) void ecs_rows_kernel(EcsArgs a, double* l_ref,
-                                                               double* ll_ref, double* r,
-                                                               double* w) {
+// One wavefront per row: l_ref, ll_ref, r, w (with the sign of -y).
+__global__ __launch_bounds__(kEcsThreads) void ecs_rows_kernel(EcsArgs a, EcsRow* rows) {
   const int ln = threadIdx.x % NAX_WAVE;
   const int64_t waves = (int64_t)gridDim.x * (kEcsThreads / NAX_WAVE);
   for (int64_t i = (int64_t)blockIdx.x * (kEcsThreads / NAX_WAVE) + threadIdx.x / NAX_WAVE;
@@ -85,10 +90,13 @@ __global__ __launch_bounds__(kEcsThreads) void ecs_rows_kernel(EcsArgs a, double
     if (ln == 0) {
       const double y = a.y[i];
       const double p = expit(s);
-      l_ref[i] = s;
-      ll_ref[i] = bern_ll(s, y);
-      r[i] = y - p;
-      w[i] = a.proxy == 2 ? p * (1.0 - p) : 0.0;
+      const double w = a.proxy == 2 ? p * (1.0 - p) : 0.0;
+      EcsRow q;
+      q.l_ref = s;
+      q.ll_ref = bern_ll(s, y);
+      q.r = y - p;
+      q.w_y = y > 0.5 ? -w : w;
+      rows[i] = q;
     }
   }
 }
@@ -102,9 +110,9 @@ __global__ __launch_bounds__(kEcsThreads) void ecs_grad_sum_kernel(EcsArgs a, do
   if (k > a.d) return;
   double s = 0;
   if (k < a.d)
-    for (int64_t i = i0; i < i1; ++i) s += a.r[i] * (double)a.x[i * a.d + k];
+    for (int64_t i = i0; i < i1; ++i) s += a.rows[i].r * (double)a.x[i * a.d + k];
   else
-    for (int64_t i = i0; i < i1; ++i) s += a.ll_ref[i];
+    for (int64_t i = i0; i < i1; ++i) s += a.rows[i].ll_ref;
   part[(int64_t)blockIdx.y * (a.d + 1) + k] = s;
 }
 
@@ -125,7 +133,7 @@ __global__ __launch_bounds__(kEcsThreads) void ecs_hess_kernel(EcsArgs a, double
       const int rr = e / 64, cc = e % 64;
       const int64_t row = rb + rr;
       const bool ok = row < r1;
-      const float wr = ok ? (float)a.w[row] : 0.0f;
+      const float wr = ok ? (float)ecs_w(a.rows[row]) : 0.0f;
       xi[rr][cc] = (ok && i0 + cc < a.d) ? wr * a.x[row * a.d + i0 + cc] : 0.0f;
       xj[rr][cc] = (ok && j0 + cc < a.d) ? a.x[row * a.d + j0 + cc] : 0.0f;
     }
@@ -229,16 +237,17 @@ __global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel
     double df = 0, cc = 0;
     if (ln < ROWS && j < m) {
       const int64_t i = idx[j];
-      const double l = mine, y = a.y[i];
-      const double ll = bern_ll(l, y);
-      const double dy = y - expit(l);
+      const double l = mine;
       if (a.proxy) {
-        const double dl = l - a.l_ref[i];
-        df = ll - (a.ll_ref[i] + a.r[i] * dl - 0.5 * a.w[i] * dl * dl);
-        cc = dy - a.r[i] + a.w[i] * dl;
+        const EcsRow q = a.rows[i];
+        const double y = ecs_y(q), w = ecs_w(q);
+        const double dl = l - q.l_ref;
+        df = bern_ll(l, y) - (q.ll_ref + q.r * dl - 0.5 * w * dl * dl);
+        cc = (y - expit(l)) - q.r + w * dl;
       } else {
-        df = ll;
-        cc = nm * dy;
+        const double y = a.y[i];
+        df = bern_ll(l, y);
+        cc = nm * (y - expit(l));
       }
       sd += df;
       sd2 += df * df;
@@ -496,19 +505,19 @@ Potential* make_ecs_logistic(const float* x, const float* y, int64_t n, int d, d
   if ((e = hipMalloc(&p->pe_new, sizeof(double) * num_chains))) return fail("hipMalloc", e);
   if ((e = hipMalloc(&p->g_new, sizeof(float) * (size_t)num_chains * Dp))) return fail("hipMalloc", e);
   if (!proxy) return p;
-  const size_t nbuf = (size_t)d + 4 * (size_t)n + d + (size_t)d * d;
+  // [rows: n records of 4 doubles][bref d][G d][H d*d]
+  const size_t nbuf = 4 * (size_t)n + d + d + (size_t)d * d;
   if ((e = hipMalloc(&p->buf, sizeof(double) * nbuf))) return fail("hipMalloc", e);
-  double* bref = p->buf;
-  double* rows = bref + d;
+  EcsRow* rows = reinterpret_cast<EcsRow*>(p->buf);
+  double* bref = p->buf + 4 * (size_t)n;
+  a.rows = rows;
   a.bref = bref;
-  a.l_ref = rows; a.ll_ref = rows + n; a.r = rows + 2 * n; a.w = rows + 3 * n;
-  double* G = rows + 4 * n;
+  double* G = bref + d;
   double* H = G + d;
   a.G = G; a.H = H;
   if ((e = hipMemcpyAsync(bref, ref, sizeof(double) * d, hipMemcpyDeviceToDevice, s)))
     return fail("hipMemcpyAsync", e);
-  hipLaunchKernelGGL(ecs_rows_kernel, dim3(2048), dim3(kEcsThreads), 0, s, a,
-                     (double*)a.l_ref, (double*)a.ll_ref, (double*)a.r, (double*)a.w);
+  hipLaunchKernelGGL(ecs_rows_kernel, dim3(2048), dim3(kEcsThreads), 0, s, a, rows);
   if ((e = hipGetLastError())) return fail("ecs_rows_kernel", e);
   double* part = nullptr;
   const size_t npart = (size_t)kEcsSplit * ((size_t)d * d > (size_t)d + 1 ? (size_t)d * d : d + 1);
