@@ -75,6 +75,15 @@ __device__ __forceinline__ double bern_ll(double l, double y) {
   return y * l - (fmax(l, 0.0) + log1p(exp(-fabs(l))));
 }
 
+// log Bernoulli(y | logits l) and y - sigmoid(l) in fp64 from one
+// e = exp(-|l|): softplus(l) = max(l, 0) + log1p(e), sigmoid(l) = 1 / (1 + e)
+// for l >= 0 and e / (1 + e) otherwise.
+__device__ __forceinline__ void bern_terms(double l, double y, double& ll, double& dy) {
+  const double e = exp(-fabs(l));
+  ll = y * l - (fmax(l, 0.0) + log1p(e));
+  dy = y - (l >= 0.0 ? 1.0 : e) / (1.0 + e);
+}
+
 // ---- set-up at the reference point ------------------------------------------
 
 // One wavefront per row: l_ref, ll_ref, r, w (with the sign of -y).
@@ -188,6 +197,7 @@ __global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel
   __shared__ double dlt[kEcsMaxD];
   __shared__ double part[2][NW][kEcsMaxD];
   __shared__ double red[NW];
+  __shared__ double shift[NW];
   int c = blockIdx.x;
   if (act) {
     if (c >= *nact) return;
@@ -204,28 +214,42 @@ __global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel
   const int wv = threadIdx.x / NAX_WAVE, ln = threadIdx.x % NAX_WAVE;
   const int nch = (d + NAX_WAVE - 1) / NAX_WAVE;
   const double nm = (double)a.n / m;
-  double s1[NCH], s2[NCH];
+  // S1 and S2 - d0 S1 in fp32, d0 = the diff of this wave's first row (so
+  // the final combination has no large cancelling terms)
+  float s1[NCH], s2[NCH];
 #pragma unroll
-  for (int t = 0; t < NCH; ++t) s1[t] = s2[t] = 0.0;
+  for (int t = 0; t < NCH; ++t) s1[t] = s2[t] = 0.0f;
   double sd = 0, sd2 = 0;  // lanes 0..ROWS-1: sum diff, sum diff^2
-  for (int j0 = wv * ROWS; j0 < m; j0 += NW * ROWS) {
+  double d0 = 0.0;
+  auto load_rows = [&](int jb, float (&dst)[ROWS][NCH]) {
     const float* xr[ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) xr[r] = a.x + (int64_t)idx[min(j0 + r, m - 1)] * d;
-    float xv[ROWS][NCH];
+    for (int r = 0; r < ROWS; ++r) xr[r] = a.x + (int64_t)idx[min(jb + r, m - 1)] * d;
+#pragma unroll
+    for (int t = 0; t < NCH; ++t) {
+      const int k = ln + t * NAX_WAVE;
+      const bool ok = t < nch && k < d;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) dst[r][t] = ok ? xr[r][k] : 0.0f;
+    }
+  };
+  float xv[ROWS][NCH];
+  int j0 = wv * ROWS;
+  if (j0 < m) load_rows(j0, xv);
+  for (bool first = true; j0 < m; j0 += NW * ROWS, first = false) {
+    // the next step's rows are in flight while this step is reduced
+    float xn[ROWS][NCH];
+    const int jn = j0 + NW * ROWS;
+    if (jn < m) load_rows(jn, xn);
     float s[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) s[r] = 0.0f;
 #pragma unroll
     for (int t = 0; t < NCH; ++t) {
       const int k = ln + t * NAX_WAVE;
-      const bool ok = t < nch && k < d;
-      const float b = beta[ok ? k : 0];
+      const float b = (t < nch && k < d) ? beta[k] : 0.0f;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        xv[r][t] = ok ? xr[r][k] : 0.0f;
-        s[r] = fmaf(xv[r][t], b, s[r]);
-      }
+      for (int r = 0; r < ROWS; ++r) s[r] = fmaf(xv[r][t], b, s[r]);
     }
     float mine = 0.0f;
 #pragma unroll
@@ -242,27 +266,35 @@ __global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel
         const EcsRow q = a.rows[i];
         const double y = ecs_y(q), w = ecs_w(q);
         const double dl = l - q.l_ref;
-        df = bern_ll(l, y) - (q.ll_ref + q.r * dl - 0.5 * w * dl * dl);
-        cc = (y - expit(l)) - q.r + w * dl;
+        double ll, dy;
+        bern_terms(l, y, ll, dy);
+        df = ll - (q.ll_ref + q.r * dl - 0.5 * w * dl * dl);
+        cc = dy - q.r + w * dl;
       } else {
-        const double y = a.y[i];
-        df = bern_ll(l, y);
-        cc = nm * (y - expit(l));
+        double dy;
+        bern_terms(l, a.y[i], df, dy);
+        cc = nm * dy;
       }
       sd += df;
       sd2 += df * df;
     }
+    if (first) d0 = __shfl(df, 0, NAX_WAVE);
+    const float ccf = (float)cc, dcf = (float)((df - d0) * cc);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      const double cr = __shfl(cc, r, NAX_WAVE);
-      const double dcr = __shfl(df * cc, r, NAX_WAVE);
+      const float cr = __shfl(ccf, r, NAX_WAVE);
+      const float dcr = __shfl(dcf, r, NAX_WAVE);
 #pragma unroll
       for (int t = 0; t < NCH; ++t) {
-        if (t < nch) {
-          s1[t] += cr * xv[r][t];
-          s2[t] += dcr * xv[r][t];
-        }
+        s1[t] = fmaf(cr, xv[r][t], s1[t]);
+        s2[t] = fmaf(dcr, xv[r][t], s2[t]);
       }
+    }
+    if (jn < m) {
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+        for (int t = 0; t < NCH; ++t) xv[r][t] = xn[r][t];
     }
   }
 #pragma unroll
@@ -273,6 +305,7 @@ __global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel
       part[1][wv][k] = s2[t];
     }
   }
+  if (ln == 0) shift[wv] = d0;
   // (H delta)_k for this thread's coordinates k = tid + 256 q, kept for the
   // gradient; H is symmetric, so column reads are coalesced
   double hk[kEcsMaxD / kEcsThreads];
@@ -319,7 +352,7 @@ __global__ __launch_bounds__(kEcsThreads, NCH <= 4 ? 4 : 2) void ecs_eval_kernel
 #pragma unroll
       for (int w = 0; w < NW; ++w) {
         t1 += part[0][w][k];
-        t2 += part[1][w][k];
+        t2 += part[1][w][k] + shift[w] * part[0][w][k];
       }
       double acc = w1 * t1 - w2 * t2;
       if (a.proxy) acc += a.G[k] + hk[q];
