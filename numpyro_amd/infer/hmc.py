@@ -403,6 +403,7 @@ class HMC(MCMCKernel):
         self._max_tree_depth = 10
         self._find_heuristic_step_size = find_heuristic_step_size
         self._regularize_mass_matrix = regularize_mass_matrix
+        self._init_strategy = init_strategy
         self._engine = None
         self._postprocess_fn = None
         self._deterministic_sites = ()
@@ -452,6 +453,11 @@ class HMC(MCMCKernel):
                 raise ValueError(
                     "Valid value of `init_params` must be provided with `potential_fn`.")
             init_params = init_to_uniform(keys, target.dim, self_split=True)
+            if self._init_strategy is not None:
+                if not hasattr(self._init_strategy, "apply"):
+                    raise ValueError("init_strategy must be init_to_value(values=...) or "
+                                     "init_to_feasible() from numpyro_amd.infer")
+                init_params = self._init_strategy.apply(init_params, target.site_sizes)
         z0 = torch.as_tensor(init_params, dtype=torch.float32)
         if z0.ndim <= 1 and num_chains == 1:
             z0 = z0.reshape(1, -1)
@@ -513,7 +519,8 @@ class NUTS(HMC):
                          dense_mass=dense_mass, target_accept_prob=target_accept_prob,
                          trajectory_length=trajectory_length,
                          find_heuristic_step_size=find_heuristic_step_size,
-                         regularize_mass_matrix=regularize_mass_matrix)
+                         regularize_mass_matrix=regularize_mass_matrix,
+                         init_strategy=init_strategy)
         self._max_tree_depth = max_tree_depth
         self._algo = "NUTS"
 

@@ -1,0 +1,50 @@
+"""Initialisation strategies for model-based kernels
+(``numpyro/infer/initialization.py``).
+
+The default draws every unconstrained latent coordinate uniformly from
+(-2, 2) (``init_to_uniform``, ``initialization.py:95-129``).
+``init_to_value(values=...)`` starts the sites named in ``values`` at those
+(constrained) values and the others as the default does
+(``initialization.py:140-156``); ``init_to_feasible`` starts every
+unconstrained coordinate at 0 (``initialization.py:132-137``).
+"""
+import torch
+
+
+class init_to_value:
+    """``init_to_value(site=None, values={})``."""
+
+    def __init__(self, site=None, values=None):
+        if site is not None:
+            raise ValueError("init_to_value is applied by the kernel: pass values= only")
+        self.values = dict(values or {})
+
+    def apply(self, z0, site_sizes):
+        """Overwrites the coordinates of the named sites in z0 [C, dim]
+        (unconstrained; positive sites through log, the inverse of their
+        exp transform)."""
+        z0 = torch.as_tensor(z0, dtype=torch.float32).clone()
+        o = 0
+        names = {name for name, _, _ in site_sizes}
+        unknown = set(self.values) - names
+        if unknown:
+            raise ValueError(f"init_to_value: no latent site named {sorted(unknown)}")
+        for name, size, tf in site_sizes:
+            if name in self.values:
+                v = torch.as_tensor(self.values[name], dtype=torch.float64).cpu()
+                if tf == "exp":
+                    v = torch.log(v)
+                z0[:, o:o + size] = v.reshape(-1, size).to(torch.float32)
+            o += size
+        return z0
+
+
+class init_to_feasible:
+    """``init_to_feasible(site=None)``: every unconstrained coordinate 0."""
+
+    def __init__(self, site=None):
+        if site is not None:
+            raise ValueError("init_to_feasible is applied by the kernel")
+
+    def apply(self, z0, site_sizes):
+        return torch.zeros_like(torch.as_tensor(z0, dtype=torch.float32))

@@ -139,14 +139,16 @@ def test_hmcecs_covtype_posterior(proxy):
     # must agree with full-data NUTS (as test/infer/test_hmc_gibbs.py:302-330
     # checks HMCECS against the true location).
     from numpyro_amd import random
-    from numpyro_amd.infer import HMCECS, MCMC, NUTS
+    from numpyro_amd.infer import HMCECS, MCMC, NUTS, init_to_value
     X, y, beta = _data(20000, 5, 7)
     Xt, yt = torch.as_tensor(X), torch.as_tensor(y)
     full = MCMC(NUTS(_covtype), num_warmup=300, num_samples=300, num_chains=4)
     full.run(random.PRNGKey(1), Xt, yt)
     ref_mean = full.get_samples()["coefs"].mean(0).cpu().numpy()
     prox = HMCECS.taylor_proxy({"coefs": torch.as_tensor(ref_mean)}) if proxy else None
-    kern = HMCECS(NUTS(_covtype), num_blocks=10, proxy=prox)
+    # examples/covtype.py:154-165: the inner kernel starts at the reference point
+    init = init_to_value(values={"coefs": torch.as_tensor(ref_mean)}) if proxy else None
+    kern = HMCECS(NUTS(_covtype, init_strategy=init), num_blocks=10, proxy=prox)
     mcmc = MCMC(kern, num_warmup=300, num_samples=300, num_chains=4)
     mcmc.run(random.PRNGKey(2), Xt, yt, 1000)
     s = mcmc.get_samples()["coefs"].cpu().numpy()
@@ -200,3 +202,18 @@ def test_plain_nuts_refuses_a_subsample_plate():
     mcmc = MCMC(NUTS(_covtype), num_warmup=5, num_samples=5)
     with pytest.raises(ValueError, match="HMCECS"):
         mcmc.run(random.PRNGKey(0), torch.as_tensor(X), torch.as_tensor(y), 50)
+
+
+def test_init_to_value_starts_chains_at_the_value():
+    from numpyro_amd import random
+    from numpyro_amd.infer import MCMC, NUTS, init_to_value
+    X, y, _ = _data(2000, 3, 11)
+    start = torch.tensor([0.3, -0.2, 0.1])
+    kern = NUTS(_covtype, init_strategy=init_to_value(values={"coefs": start}))
+    keys = np.stack([np.asarray(random.PRNGKey(i), np.uint32) for i in range(3)])
+    kern.init(keys, 10, None, (torch.as_tensor(X), torch.as_tensor(y)), {})
+    z = kern._engine.get_state().z.cpu()
+    assert torch.equal(z, start.expand(3, 3))
+    mcmc = MCMC(kern, num_warmup=50, num_samples=50, num_chains=3)
+    mcmc.run(random.PRNGKey(1), torch.as_tensor(X), torch.as_tensor(y))
+    assert mcmc.get_samples()["coefs"].shape == (150, 3)
