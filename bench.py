@@ -121,7 +121,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); chains "
-              "shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+              "would shard across GPUs with no communication", file=sys.stderr)
         sys.exit(2)
 
     import torch

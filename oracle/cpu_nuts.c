@@ -1,4 +1,4 @@
-/* CPU restatement of the logistic-regression NUTS path -- TEST / BASELINE
+/* CPU restatement of the logistic-regression NUTS path -- TEST AND CPU-BASELINE
  * INFRASTRUCTURE ONLY (the oracle's C leg, timed as the CPU baseline).
  *
  * Same algorithm as the float64 numpy oracle (oracle/hmc_util.py,
