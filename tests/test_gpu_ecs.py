@@ -217,3 +217,22 @@ def test_init_to_value_starts_chains_at_the_value():
     mcmc = MCMC(kern, num_warmup=50, num_samples=50, num_chains=3)
     mcmc.run(random.PRNGKey(1), torch.as_tensor(X), torch.as_tensor(y))
     assert mcmc.get_samples()["coefs"].shape == (150, 3)
+
+
+def test_estimated_potential_tracks_the_full_potential():
+    # test/infer/test_hmc_gibbs.py:402-441 with the logistic likelihood: at the
+    # drawn points, exp(U_full - U_estimated) stays near 1
+    from numpyro_amd import random
+    from numpyro_amd.infer import HMCECS, MCMC, NUTS
+    from numpyro_amd.models import LogisticRegression, potential_energy
+    X, y, beta = _data(2000, 3, 13)
+    Xt, yt = torch.as_tensor(X), torch.as_tensor(y)
+    kern = HMCECS(NUTS(_covtype), num_blocks=20,
+                  proxy=HMCECS.taylor_proxy({"coefs": torch.as_tensor(beta, dtype=torch.float32)}))
+    mcmc = MCMC(kern, num_warmup=200, num_samples=200, num_chains=2)
+    mcmc.run(random.PRNGKey(0), Xt, yt, 100, extra_fields=("potential_energy",))
+    pes = mcmc.get_extra_fields()["potential_energy"].double()
+    z = mcmc.get_samples()["coefs"]
+    full = potential_energy(LogisticRegression(Xt, yt), z).double()
+    r = torch.exp(full - pes)
+    assert float(r.var()) < 1.0, (float(r.mean()), float(r.var()))
