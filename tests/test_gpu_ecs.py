@@ -33,7 +33,8 @@ def _model(X, y, m, degree, ref, C, seed=0, nb=1):
 
 
 @pytest.mark.parametrize("degree", [0, 1, 2])
-@pytest.mark.parametrize("N,D,m", [(3000, 8, 100), (5000, 33, 1000), (20000, 256, 700)])
+@pytest.mark.parametrize("N,D,m", [(3000, 8, 100), (5000, 33, 1000), (20000, 256, 700),
+                                   (4000, 130, 300), (3000, 300, 200)])
 def test_subsampled_potential_vs_oracle(degree, N, D, m):
     from numpyro_amd.models import potential_energy
     X, y, beta = _data(N, D)
