@@ -456,10 +456,10 @@ struct EcsLogistic : Potential {
   float* g_new = nullptr;
   double* buf = nullptr;  // bref, per-row scalars, G, H
   ~EcsLogistic() override {
-    hipFree(prop);
-    hipFree(pe_new);
-    hipFree(g_new);
-    hipFree(buf);
+    (void)hipFree(prop);
+    (void)hipFree(pe_new);
+    (void)hipFree(g_new);
+    (void)hipFree(buf);
   }
   int dim() const override { return a.d; }
   const char* name() const override { return "ecs_eval_kernel"; }
@@ -573,8 +573,8 @@ Potential* make_ecs_logistic(const float* x, const float* y, int64_t n, int d, d
     if ((e = hipMemsetAsync(H, 0, sizeof(double) * d * d, s))) return fail("hipMemsetAsync", e);
   }
   if ((e = hipGetLastError())) return fail("ecs set-up", e);
-  hipFreeAsync(part, s);
-  hipFreeAsync(gl, s);
+  const hipError_t f1 = hipFreeAsync(part, s), f2 = hipFreeAsync(gl, s);
+  if ((e = f1 != hipSuccess ? f1 : f2)) return fail("hipFreeAsync", e);
   if ((e = hipStreamSynchronize(s))) return fail("ecs set-up", e);
   a.LL = LL;
   return p;
