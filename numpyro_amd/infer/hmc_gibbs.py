@@ -10,8 +10,6 @@ runs on the chain's (possibly new) subsample.  Models: logistic regression
 with a subsample plate over the rows of X (``examples/covtype.py:66-71`` with
 ``subsample_size``), traced by the model compiler.
 """
-import ctypes as C
-
 import numpy as np
 import torch
 
@@ -74,7 +72,7 @@ class HMCECS(MCMCKernel):
 
     @property
     def default_fields(self):
-        return ("z", "diverging")
+        return ("z",)
 
     @property
     def _engine(self):

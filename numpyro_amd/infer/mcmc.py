@@ -148,7 +148,11 @@ class MCMC:
         size = (upper - lower) // self.thinning if size is None else size
         i0 = int(eng.get_state().i.max())
         start_idx = i0 + lower + (upper - lower) % self.thinning
-        dev_fields = tuple(f for f in collect_fields if f != "z") + ("z",)
+        # HMCECS keeps the inner kernel's state under hmc_state
+        # (HMCECSState, numpyro/infer/hmc_gibbs.py:495-500)
+        ren = {f: f[len("hmc_state."):] for f in collect_fields
+               if f.startswith("hmc_state.") and getattr(self.sampler, "_gibbs_sites", None)}
+        dev_fields = tuple(ren.get(f, f) for f in collect_fields if f != "z") + ("z",)
         out = eng.run(i0 + upper, collect_fields=dev_fields, start_idx=start_idx,
                       thinning=self.thinning, size=size)
         self._grad_evals = eng.grad_evals
@@ -165,8 +169,8 @@ class MCMC:
                     for s in remove_sites:
                         z.pop(s, None)
                 states[f] = z
-            elif f in out:
-                states[f] = out[f]
+            elif ren.get(f, f) in out:
+                states[f] = out[ren.get(f, f)]
             else:
                 raise ValueError(f"field {f!r} cannot be collected")
         self._states = states

@@ -231,8 +231,8 @@ def test_estimated_potential_tracks_the_full_potential():
     kern = HMCECS(NUTS(_covtype), num_blocks=20,
                   proxy=HMCECS.taylor_proxy({"coefs": torch.as_tensor(beta, dtype=torch.float32)}))
     mcmc = MCMC(kern, num_warmup=200, num_samples=200, num_chains=2)
-    mcmc.run(random.PRNGKey(0), Xt, yt, 100, extra_fields=("potential_energy",))
-    pes = mcmc.get_extra_fields()["potential_energy"].double()
+    mcmc.run(random.PRNGKey(0), Xt, yt, 100, extra_fields=("hmc_state.potential_energy",))
+    pes = mcmc.get_extra_fields()["hmc_state.potential_energy"].double()
     z = mcmc.get_samples()["coefs"]
     full = potential_energy(LogisticRegression(Xt, yt), z).double()
     r = torch.exp(full - pes)
