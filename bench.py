@@ -2,14 +2,21 @@
 
 A "step" is one NUTS iteration of every chain (each chain's full trajectory:
 momentum draw, iterative tree, multinomial proposal) on the synthetic
-2,000,000 x 256 design resident in HBM.  --warmup W untimed iterations run
-first (dual averaging + diagonal Welford on the window schedule of W), then K
-timed sampling iterations.  value = gradient evaluations per second over the
-timed region (useful leapfrog steps summed over chains); min-ESS/s is
-computed from the K timed draws of every chain.  The BF16-MFMA form is the
-measured line; the plain FP32 form (same adapted state, same kernels
-otherwise) is timed beside it, and the CPU baseline (oracle/libcpu_nuts.so,
-a C/OpenMP restatement of the same NUTS path) on a bounded sample.
+2,000,000 x 256 design resident in HBM.  Set-up (untimed): a Laplace
+approximation of the posterior (Newton iterations on U in float64) gives
+each chain's initial point (the mode plus a uniform(-2, 2) jitter in units
+of the Laplace standard deviation, passed as init_params) and the initial
+diagonal inverse_mass_matrix (the Laplace variances); the sampler runs with
+find_heuristic_step_size=True.  --warmup W untimed iterations follow (dual
+averaging + diagonal Welford on the window schedule of W), then K timed
+sampling iterations.  value = gradient evaluations per second over the timed
+region (useful leapfrog steps summed over chains); min-ESS/s is computed
+from the K timed draws of every chain.  The BF16-MFMA form is the measured
+line; the plain FP32 form (same adapted state, same kernels otherwise) is
+timed beside it, and the CPU baseline (oracle/libcpu_nuts.so, a C/OpenMP
+restatement of the same NUTS path) on a bounded sample.  A run whose timed
+transitions mostly diverge (or accept almost nothing) prints its line with
+"valid": false and exits with status 3.
 """
 import argparse
 import json
@@ -29,11 +36,12 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True):
+def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True, imm=None,
+             heuristic=False):
     import torch
     from numpyro_amd.infer.hmc import Engine
-    eng = Engine(model, C, "NUTS", num_warmup)
-    eng.init(z0, keys)
+    eng = Engine(model, C, "NUTS", num_warmup, find_heuristic_step_size=heuristic)
+    eng.init(z0, keys, imm)
     t_start = time.perf_counter()
     eng.set_progress(lambda ticks, pending: log(
         f"  {getattr(model, 'x_dtype', type(model).__name__)}: {ticks} ticks/launches, "
@@ -76,6 +84,12 @@ def min_ess(z):
     ess, rhat = device_diagnostics(z)
     rhat = rhat.cpu().numpy() if z.shape[1] >= 4 else np.full(z.shape[2], np.nan)
     return float(ess.min()), float(np.nanmax(rhat))
+
+
+def degenerate(div, acc, transitions):
+    """True when more than half of the timed transitions diverged or the
+    mean acceptance probability is below 0.05."""
+    return div > 0.5 * transitions or not acc >= 0.05
 
 
 def cpu_baseline(X, y, z0, keys, budget_s, threads):
@@ -142,21 +156,29 @@ def main():
     N, D, C = cfg["N"], cfg["D"], cfg["chains"]
     log(f"workload {args.config}: N={N} D={D} chains={C} x_dtype={args.x_dtype}")
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
-    z0 = torch.zeros((C, D), dtype=torch.float32, device="cuda")  # init_to_value(0)
+    t0 = time.perf_counter()
+    z0, imm = workloads.laplace_init(X, y, C)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"laplace init: {setup_s:.1f}s, inverse mass diagonal in "
+        f"[{float(imm.min()):.3g}, {float(imm.max()):.3g}]")
 
     model = LogisticRegression(
         X, y, x_dtype=args.x_dtype,
         kernel=args.bf16_kernel if args.x_dtype == "bf16" else "auto")
-    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys,
+                                               imm=imm, heuristic=True)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
     div = int(out["diverging"].sum())
     acc = float(out["accept_prob"].float().mean())
     mess, rhat = min_ess(out["z"])
     gps = ge / wall
+    valid = not degenerate(div, acc, C * args.steps)
     log(f"{args.x_dtype}: warmup {warm_s:.1f}s, timed {wall:.2f}s, grad evals {ge}, "
         f"{gps:.4g}/s, min ESS {mess:.1f}, max R-hat {rhat:.3f}, mean tree {ns.mean():.1f}, "
-        f"accept {acc:.3f}, divergences {div}")
+        f"accept {acc:.3f}, divergences {div}, step size "
+        f"{float(eng.get_state().adapt_state.step_size.mean()):.3g}")
     # Every tick evaluates only the chains that asked for a gradient (the
     # compacted request list), so the algorithmic work of the timed launches
     # is 4 N D per evaluation times the timed gradient evaluations.
@@ -167,7 +189,7 @@ def main():
     post_state = eng.get_state()
 
     fp32 = None
-    if args.fp32_steps > 0 and args.x_dtype != "fp32":
+    if args.fp32_steps > 0 and args.x_dtype != "fp32" and valid:
         m32 = LogisticRegression(X, y, x_dtype="fp32", kernel="mfma")
         e32, o32, w32, _, k32, n32 = run_form(m32, C, args.warmup, args.fp32_steps, z0, keys,
                                               state=post_state)
@@ -213,8 +235,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"logistic NUTS {args.config}: N={N}, D={D}, chains={C}, "
                                f"X {args.x_dtype}, max_tree_depth 10, target_accept 0.8, "
-                               f"diag mass, init 0", "global_batch": C, "seq_len": N,
-                   "parallelism": "chains"},
+                               f"diag mass", "global_batch": C, "seq_len": N,
+                   "parallelism": "chains",
+                   "init_params": "Laplace mode + Laplace sd * uniform(-2, 2) per chain",
+                   "inverse_mass_matrix": "Laplace variances (diagonal)",
+                   "find_heuristic_step_size": True},
+        "valid": valid,
+        "setup_s": setup_s,
         "min_ess_per_s": mess / wall,
         "min_ess": mess,
         "min_ess_per_iteration": mess / args.steps,
@@ -235,6 +262,10 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if not valid:
+        log(f"degenerate run: {div} of {C * args.steps} timed transitions diverged, mean "
+            f"accept {acc:.3f}")
+        sys.exit(3)
 
 
 def bench_sv(args):
@@ -287,7 +318,8 @@ def bench_hier(args):
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
     z0 = torch.zeros((C, model.dim), dtype=torch.float32, device="cuda")
     eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
-    mess, rhat = min_ess(out["z"][:, :, :4])
+    per = group_diagnostics(out["z"], model.dims, 4)
+    mess, rhat = per["min_ess"], per["max_rhat"]
     ge = eng.grad_evals
     line = {"metric": "min-ESS/sec + grad evals/sec, ragged hierarchical logistic NUTS",
             "value": ge / wall, "unit": "grad evals/s", "n_gpus": 1, "steps": args.steps,
@@ -297,7 +329,10 @@ def bench_hier(args):
             "config": {"workload": f"{G} hierarchical logistic regressions, N_i U[50,5000], "
                                    f"D_i U[4,64], 4 chains each", "global_batch": C,
                        "seq_len": int(sum(x.shape[0] for x in Xs)), "parallelism": "chains"},
-            "min_ess_per_s_first4coords": mess / wall, "max_rhat_first4coords": rhat,
+            "min_ess_per_s": mess / wall, "min_ess": mess, "max_rhat": rhat,
+            "diagnostics": "per model over its own 4 chains and D_i + 2 coordinates; "
+                           "min ESS / max split R-hat over the 1024 models",
+            "groups_rhat_above_1.01": per["n_bad"], "worst_groups": per["worst"],
             "mean_tree_size": float(out["num_steps"].float().mean()),
             "mean_accept_prob": float(out["accept_prob"].float().mean()),
             "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
@@ -306,6 +341,29 @@ def bench_hier(args):
             "hbm_GBps_at_grad_rate": model.algorithmic_bytes(C) * ge / C / wall / 1e9}
     log(json.dumps(line))
     print(json.dumps(line), flush=True)
+
+
+def group_diagnostics(z, dims, K):
+    """ESS and split R-hat of each model over its own K chains: draws z
+    [G*K, S, Dmax] are regrouped to [K, S, G*Dmax] so that every (model,
+    coordinate) pair is one parameter of the diagnostics kernel; entries past
+    a model's D_i + 2 coordinates (zero padding) are dropped."""
+    import torch
+    from numpyro_amd.diagnostics import device_diagnostics
+    G = len(dims)
+    CK, S, Dm = z.shape
+    zz = z.reshape(G, K, S, Dm).permute(1, 2, 0, 3).reshape(K, S, G * Dm).contiguous()
+    ess, rhat = device_diagnostics(zz)
+    ess = ess.reshape(G, Dm).cpu().numpy()
+    rhat = rhat.reshape(G, Dm).cpu().numpy()
+    mask = np.arange(Dm)[None, :] < (np.asarray(dims)[:, None] + 2)
+    ess_g = np.where(mask, ess, np.inf).min(1)
+    rhat_g = np.where(mask, rhat, -np.inf).max(1)
+    order = np.argsort(-rhat_g)[:5]
+    return {"min_ess": float(ess_g.min()), "max_rhat": float(rhat_g.max()),
+            "n_bad": int((rhat_g > 1.01).sum()),
+            "worst": [{"group": int(g), "D": int(dims[g]), "max_rhat": float(rhat_g[g]),
+                       "min_ess": float(ess_g[g])} for g in order]}
 
 
 def bench_ecs(args):

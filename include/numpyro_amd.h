@@ -100,6 +100,27 @@ typedef struct {
   uint32_t* rng_key; /* [C, 2] */
 } nax_state_view;
 
+/* Host view of the rest of HMCAdaptState (hmc_util.py:16-30): the
+ * dual-averaging state ss_state = (x_t, x_avg, g_avg, t, prox_center)
+ * (dual_averaging, hmc_util.py:97-130), the Welford state mm_state =
+ * (mean, m2, n) (welford_covariance, hmc_util.py:160-196), window_idx and
+ * the adapter's rng_key (hmc_util.py:565, 655-656).  Host arrays are [C];
+ * d_mm_mean is [C, dim] fp64 and d_mm_m2 [C, dim] fp64, or [C, dim, dim]
+ * fp64 under dense_mass (m2[i][j] = sum of delta_post_i * delta_pre_j).
+ * Any pointer may be NULL (that part is not copied). */
+typedef struct {
+  double* ss_x_t;
+  double* ss_x_avg;
+  double* ss_g_avg;
+  int32_t* ss_t;
+  double* ss_prox_center;
+  double* d_mm_mean;
+  double* d_mm_m2;
+  int32_t* mm_n;
+  int32_t* window_idx;
+  uint32_t* rng_key; /* [C, 2] */
+} nax_adapt_view;
+
 const char* nax_last_error(void);
 
 /* Replaces hmc(potential_fn, algo=...) (hmc.py:113) + init_kernel's static
@@ -225,6 +246,13 @@ int nax_exchange(nax_sampler* h, float* d_z_out, const double* d_pe_in, const fl
  * per-chain state out of / into the sampler. */
 int nax_get_state(nax_sampler* h, const nax_state_view* v);
 int nax_set_state(nax_sampler* h, const nax_state_view* v);
+
+/* HMCAdaptState's ss_state / mm_state / window_idx / rng_key out of / into
+ * the sampler, so that a state taken mid-warmup resumes exactly (the
+ * reference's functional state, hmc.py:459-530 and mcmc.py:558-587, carries
+ * them in hmc_state.adapt_state). */
+int nax_get_adapt_state(nax_sampler* h, const nax_adapt_view* v);
+int nax_set_adapt_state(nax_sampler* h, const nax_adapt_view* v);
 
 /* Potential and gradient of the built-in model at d_z [count, ld]:
  * potential_energy + value_and_grad (util.py:333-358, hmc_util.py:242-252). */

@@ -48,6 +48,15 @@ class nax_state_view(C.Structure):
     ]
 
 
+class nax_adapt_view(C.Structure):
+    _fields_ = [
+        ("ss_x_t", C.c_void_p), ("ss_x_avg", C.c_void_p), ("ss_g_avg", C.c_void_p),
+        ("ss_t", C.c_void_p), ("ss_prox_center", C.c_void_p), ("d_mm_mean", C.c_void_p),
+        ("d_mm_m2", C.c_void_p), ("mm_n", C.c_void_p), ("window_idx", C.c_void_p),
+        ("rng_key", C.c_void_p),
+    ]
+
+
 PROGRESS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int32)
 
 # name -> (restype, argtypes)
@@ -82,6 +91,8 @@ SIGNATURES = {
     "nax_exchange": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nax_get_state": (C.c_int, [C.c_void_p, C.POINTER(nax_state_view)]),
     "nax_set_state": (C.c_int, [C.c_void_p, C.POINTER(nax_state_view)]),
+    "nax_get_adapt_state": (C.c_int, [C.c_void_p, C.POINTER(nax_adapt_view)]),
+    "nax_set_adapt_state": (C.c_int, [C.c_void_p, C.POINTER(nax_adapt_view)]),
     "nax_potential_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                      C.c_void_p, C.c_void_p]),
     "nax_leapfrog": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -1,32 +1,53 @@
-"""Builds libnumpyro_amd.so for gfx950 in-tree with hipcc."""
+"""Builds libnumpyro_amd.so for gfx950 in-tree with hipcc.
+
+Each translation unit compiles to its own object (in parallel, objects under
+numpyro_amd/_build/), then one hipcc line links the shared library.  The
+units share host-level interfaces only (nax_internal.h), so no relocatable
+device code is needed."""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libnumpyro_amd.so")
 SOURCES = ["sampler.hip", "logistic.hip", "logistic_mfma.hip", "logistic_chain.hip", "sv.hip",
            "hier.hip", "diagnostics.hip", "ecs.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off (v_pk_{add,mul,fma}_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _run(cmd, what):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libnumpyro_amd.so")
+        raise RuntimeError(f"hipcc failed: {what}")
+    return r
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    objs = [os.path.join(OBJ, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [[HIPCC] + FLAGS + ["-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(SOURCES, objs)]
+    if verbose:
+        for c in cmds:
+            print(" ".join(c), flush=True)
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda c: _run(c, c[-3]), cmds))
+    tmp = OUT + ".tmp"
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
+    if verbose:
+        print(" ".join(link), flush=True)
+    _run(link, "link libnumpyro_amd.so")
     os.replace(tmp, OUT)
     return OUT
 

@@ -323,8 +323,11 @@ def _normal_scale(site, name):
 
 def _match_logistic(lat, obs):
     """b ~ Normal(0, s) [D], optionally a ~ Normal(0, s) scalar intercept
-    with the same s, y ~ Bernoulli(logits = X b [+ a]).  The intercept is the
-    kernel's last coefficient on an appended column of ones."""
+    with the same s, y ~ Bernoulli(logits = X b [+ a]).  The intercept is a
+    coefficient of the kernel on a column of ones, placed where the
+    reference's flattening puts its site: ravel_pytree orders the latent dict
+    by sorted site name, so the ones column comes first when the intercept's
+    name sorts before the coefficients' and last otherwise."""
     if len(lat) not in (1, 2) or len(obs) != 1:
         raise _NoMatch
     (oname, osite), = obs.items()
@@ -354,8 +357,13 @@ def _match_logistic(lat, obs):
     if aname is not None:
         if _normal_scale(lat[aname], aname) != s:
             raise _NoMatch  # one prior scale for all coefficients
-        X = torch.cat([X, torch.ones((X.shape[0], 1), dtype=X.dtype, device=X.device)], 1)
-        sites = sites + ((aname, 1, None),)
+        ones = torch.ones((X.shape[0], 1), dtype=X.dtype, device=X.device)
+        if aname < bname:
+            X = torch.cat([ones, X], 1)
+            sites = ((aname, 1, None),) + sites
+        else:
+            X = torch.cat([X, ones], 1)
+            sites = sites + ((aname, 1, None),)
     return "logistic", dict(X=X, y=_cval(osite["value"]).reshape(-1), prior_scale=s,
                             sites=sites)
 

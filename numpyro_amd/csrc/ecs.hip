@@ -412,7 +412,10 @@ __global__ __launch_bounds__(kEcsThreads) void ecs_accept_kernel(
   if (c >= C) return;
   const Key rk{keys[2 * c], keys[2 * c + 1]};
   const double pe = st[c].pe;
-  const double p = fmin(exp(pe - pe_new[c]), 1.0);
+  // clip(exp(pe - pe_new), None, 1) keeps a NaN (a NaN proposal energy),
+  // and bernoulli(key, NaN) is false: such a proposal is rejected
+  const double x = exp(pe - pe_new[c]);
+  const double p = x > 1.0 ? 1.0 : x;
   const bool acc = bernoulli(rk, p, part);
   if (acc) {
     for (int j = threadIdx.x; j < m; j += kEcsThreads)

@@ -144,6 +144,36 @@ __global__ void dense_io(int C, int D, int Dp, float* blk, float* buf, int to_bl
   else *u = *b;
 }
 
+// Welford state <-> host-view buffers: mean [C][D] and the diagonal m2
+// [C][D] live in wel [C][3][Dp] (mean, m2, scratch), fp64.
+__global__ void welford_io(int C, int D, int Dp, double* wel, double* mean, double* m2,
+                           int to_dev) {
+  const int c = blockIdx.y;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C || d >= D) return;
+  double* w = wel + (size_t)c * 3 * Dp;
+  if (to_dev) {
+    if (mean) w[d] = mean[(size_t)c * D + d];
+    if (m2) w[Dp + d] = m2[(size_t)c * D + d];
+  } else {
+    if (mean) mean[(size_t)c * D + d] = w[d];
+    if (m2) m2[(size_t)c * D + d] = w[Dp + d];
+  }
+}
+
+// Dense Welford co-moments: buf [C][D][D] <-> dm2 [C][Dp][Dp] (same
+// orientation, m2[i][j] = sum dpost_i dpre_j).
+__global__ void dense_m2_io(int C, int D, int Dp, double* dm2, double* buf, int to_dev) {
+  const int c = blockIdx.z;
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C || i >= D || j >= D) return;
+  double* b = dm2 + (size_t)c * Dp * Dp + (size_t)i * Dp + j;
+  double* u = buf + ((size_t)c * D + i) * D + j;
+  if (to_dev) *b = *u;
+  else *u = *b;
+}
+
 // ---- standalone leapfrog (parity entry point) ------------------------------------
 __global__ void lf_half_kernel(int C, int D, int ldz, float* z, float* r, const float* g,
                                const float* imm, float eps, float* zreq, int ldq) {
@@ -787,6 +817,72 @@ extern "C" int nax_set_state(nax_sampler* h, const nax_state_view* v) {
   if (!h || !v) { set_error("null argument"); return NAX_EINVAL; }
   if (!h->initialized) { set_error("nax_set_state before nax_init"); return NAX_ESTATE; }
   return state_io(h, v, true);
+}
+
+static int adapt_io(nax_sampler* h, const nax_adapt_view* v, bool to_dev) {
+  const EngineCfg& e = h->cfg;
+  if (v->d_mm_mean || (v->d_mm_m2 && !e.dense)) {
+    hipLaunchKernelGGL(welford_io, dim3((e.D + 127) / 128, e.C), dim3(128), 0, h->stream, e.C,
+                       e.D, e.Dp, h->wel, v->d_mm_mean, e.dense ? nullptr : v->d_mm_m2,
+                       to_dev ? 1 : 0);
+    NAX_HIP(hipGetLastError());
+  }
+  if (v->d_mm_m2 && e.dense) {
+    hipLaunchKernelGGL(dense_m2_io, dim3((e.D + 127) / 128, e.D, e.C), dim3(128), 0, h->stream,
+                       e.C, e.D, e.Dp, e.dm2, v->d_mm_m2, to_dev ? 1 : 0);
+    NAX_HIP(hipGetLastError());
+  }
+  std::vector<ChainState> hs(e.C);
+  NAX_HIP(hipMemcpyAsync(hs.data(), h->st, sizeof(ChainState) * e.C, hipMemcpyDeviceToHost,
+                         h->stream));
+  NAX_HIP(hipStreamSynchronize(h->stream));
+  for (int c = 0; c < e.C; ++c) {
+    ChainState& s = hs[c];
+    if (to_dev) {
+      if (v->ss_x_t) s.da_x = v->ss_x_t[c];
+      if (v->ss_x_avg) s.da_xavg = v->ss_x_avg[c];
+      if (v->ss_g_avg) s.da_gavg = v->ss_g_avg[c];
+      if (v->ss_t) s.da_t = v->ss_t[c];
+      if (v->ss_prox_center) s.da_prox = v->ss_prox_center[c];
+      if (v->mm_n) s.welford_n = v->mm_n[c];
+      if (v->window_idx) s.window_idx = v->window_idx[c];
+      if (v->rng_key) { s.adapt_key[0] = v->rng_key[2 * c]; s.adapt_key[1] = v->rng_key[2 * c + 1]; }
+    } else {
+      if (v->ss_x_t) v->ss_x_t[c] = s.da_x;
+      if (v->ss_x_avg) v->ss_x_avg[c] = s.da_xavg;
+      if (v->ss_g_avg) v->ss_g_avg[c] = s.da_gavg;
+      if (v->ss_t) v->ss_t[c] = s.da_t;
+      if (v->ss_prox_center) v->ss_prox_center[c] = s.da_prox;
+      if (v->mm_n) v->mm_n[c] = s.welford_n;
+      if (v->window_idx) v->window_idx[c] = s.window_idx;
+      if (v->rng_key) { v->rng_key[2 * c] = s.adapt_key[0]; v->rng_key[2 * c + 1] = s.adapt_key[1]; }
+    }
+  }
+  if (to_dev) {
+    for (int c = 0; c < e.C; ++c)
+      if (hs[c].window_idx < 0 || hs[c].window_idx > e.num_windows || hs[c].welford_n < 0 ||
+          hs[c].da_t < 0) {
+        set_error("nax_set_adapt_state: window_idx, Welford count or dual-averaging t out of "
+                  "range");
+        return NAX_EINVAL;
+      }
+    NAX_HIP(hipMemcpyAsync(h->st, hs.data(), sizeof(ChainState) * e.C, hipMemcpyHostToDevice,
+                           h->stream));
+  }
+  NAX_HIP(hipStreamSynchronize(h->stream));
+  return NAX_OK;
+}
+
+extern "C" int nax_get_adapt_state(nax_sampler* h, const nax_adapt_view* v) {
+  if (!h || !v) { set_error("null argument"); return NAX_EINVAL; }
+  if (!h->initialized) { set_error("no state before nax_init"); return NAX_ESTATE; }
+  return adapt_io(h, v, false);
+}
+
+extern "C" int nax_set_adapt_state(nax_sampler* h, const nax_adapt_view* v) {
+  if (!h || !v) { set_error("null argument"); return NAX_EINVAL; }
+  if (!h->initialized) { set_error("nax_set_adapt_state before nax_init"); return NAX_ESTATE; }
+  return adapt_io(h, v, true);
 }
 
 extern "C" int nax_potential_grad(nax_sampler* h, const float* z, int32_t ld, int32_t count,

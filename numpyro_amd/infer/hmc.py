@@ -165,6 +165,8 @@ class Engine:
         col.size = int(size)
         if size > 0:
             for f in collect_fields:
+                if f == "_gibbs_accept_prob" and self.gibbs:
+                    continue
                 if f == "z":
                     out[f] = torch.zeros((self.C, size, self.D), dtype=torch.float32, device="cuda")
                     col.d_z = out[f].data_ptr()
@@ -173,11 +175,15 @@ class Engine:
                     setattr(col, _COLLECT_PTR[f], out[f].data_ptr())
                 else:
                     raise ValueError(f"cannot collect field {f!r}")
-        gib = None
+        gib = gacc = None
         if self.gibbs and size > 0 and "z" in collect_fields:
             # the subsample of each collected iteration (HMCECS's Gibbs site)
             gib = torch.zeros((self.C, size, self.model.m), dtype=torch.int32, device="cuda")
             out["_gibbs"] = gib
+        if self.gibbs and size > 0 and "_gibbs_accept_prob" in collect_fields:
+            # HMCECSState.accept_prob of each collected iteration
+            gacc = torch.zeros((self.C, size), dtype=torch.float32, device="cuda")
+            out["_gibbs_accept_prob"] = gacc
         torch.cuda.synchronize()
         if self.external:
             self.grad_evals = self._advance_external(target, C.byref(col))
@@ -192,10 +198,12 @@ class Engine:
                 _lib.check(self.lib.nax_run(self.h, t + 1, C.byref(col), C.byref(ge)), "nax_run")
                 total += ge.value
                 rel = t - int(start_idx)
-                if gib is not None and rel >= 0 and rel % thinning == thinning - 1 \
-                        and rel // thinning < size:
-                    gib[:, rel // thinning].copy_(self.model.idx)
-                    torch.cuda.synchronize()  # before the next update rewrites idx
+                if rel >= 0 and rel % thinning == thinning - 1 and rel // thinning < size:
+                    if gib is not None:
+                        gib[:, rel // thinning].copy_(self.model.idx)
+                    if gacc is not None:
+                        gacc[:, rel // thinning].copy_(self.gibbs_accept)
+                    torch.cuda.synchronize()  # before the next update rewrites them
             self.grad_evals = total
         else:
             ge = C.c_int64()
@@ -251,14 +259,55 @@ class Engine:
         # mass_matrix_sqrt_inv: 1 / sqrt(M) (diagonal) or tril_inv = S^-1 (dense)
         msi = (torch.linalg.inv(t["msqrt"].double()).float() if self.dense
                else 1.0 / t["msqrt"])
+        ss_state, mm_state, window_idx, akey = self._get_adapt()
         adapt = HMCAdaptState(torch.from_numpy(h["ss"]), t["imm"], t["msqrt"],
-                              msi, None, None, None, None)
+                              msi, ss_state, mm_state, window_idx, akey)
         return HMCState(torch.from_numpy(h["i"]), t["z"], t["z_grad"],
                         torch.from_numpy(h["pe"]), torch.from_numpy(h["energy"]), None,
                         self.cfg.trajectory_length or None, torch.from_numpy(h["ns"]),
                         torch.from_numpy(h["ap"]), torch.from_numpy(h["map"]),
                         torch.from_numpy(h["div"]).bool(), adapt,
                         h["key"].reshape(Cn, 2))
+
+    def _get_adapt(self):
+        """(ss_state, mm_state, window_idx, rng_key) of HMCAdaptState
+        (hmc_util.py:16-30): ss_state = (x_t, x_avg, g_avg, t, prox_center)
+        [C] each, mm_state = (mean [C, D], m2 [C, D] or [C, D, D], n [C])."""
+        Cn, D = self.C, self.D
+        f = {k: np.zeros(Cn) for k in ("x", "xa", "ga", "pc")}
+        ints = {k: np.zeros(Cn, np.int32) for k in ("t", "n", "w")}
+        key = np.zeros(2 * Cn, np.uint32)
+        mean = torch.empty((Cn, D), dtype=torch.float64, device="cuda")
+        m2 = torch.empty((Cn, D, D) if self.dense else (Cn, D), dtype=torch.float64,
+                         device="cuda")
+        v = _lib.nax_adapt_view(f["x"].ctypes.data, f["xa"].ctypes.data, f["ga"].ctypes.data,
+                                ints["t"].ctypes.data, f["pc"].ctypes.data, mean.data_ptr(),
+                                m2.data_ptr(), ints["n"].ctypes.data, ints["w"].ctypes.data,
+                                key.ctypes.data)
+        torch.cuda.synchronize()
+        _lib.check(self.lib.nax_get_adapt_state(self.h, C.byref(v)), "nax_get_adapt_state")
+        t = torch.from_numpy
+        ss = (t(f["x"]), t(f["xa"]), t(f["ga"]), t(ints["t"]), t(f["pc"]))
+        return ss, (mean, m2, t(ints["n"])), t(ints["w"]), key.reshape(Cn, 2)
+
+    def _set_adapt(self, adapt):
+        Cn, D = self.C, self.D
+        arr = lambda x, dt: np.ascontiguousarray(np.broadcast_to(  # noqa: E731
+            np.asarray(torch.as_tensor(x).cpu().numpy(), dt), (Cn,)))
+        ss, mm, widx, akey = adapt.ss_state, adapt.mm_state, adapt.window_idx, adapt.rng_key
+        x, xa, ga, tt, pc = (arr(ss[0], np.float64), arr(ss[1], np.float64),
+                             arr(ss[2], np.float64), arr(ss[3], np.int32), arr(ss[4], np.float64))
+        shape2 = (Cn, D, D) if self.dense else (Cn, D)
+        mean = torch.as_tensor(mm[0], dtype=torch.float64).to("cuda").expand(Cn, D).contiguous()
+        m2 = torch.as_tensor(mm[1], dtype=torch.float64).to("cuda").expand(shape2).contiguous()
+        n, w = arr(mm[2], np.int32), arr(widx, np.int32)
+        key = np.ascontiguousarray(np.broadcast_to(
+            np.asarray(akey, np.uint32).reshape(-1, 2), (Cn, 2))).ravel()
+        v = _lib.nax_adapt_view(x.ctypes.data, xa.ctypes.data, ga.ctypes.data, tt.ctypes.data,
+                                pc.ctypes.data, mean.data_ptr(), m2.data_ptr(), n.ctypes.data,
+                                w.ctypes.data, key.ctypes.data)
+        torch.cuda.synchronize()
+        _lib.check(self.lib.nax_set_adapt_state(self.h, C.byref(v)), "nax_set_adapt_state")
 
     def set_state(self, st, rng_key=None):
         Cn = self.C
@@ -287,6 +336,8 @@ class Engine:
                                 keys.ctypes.data)
         torch.cuda.synchronize()
         _lib.check(self.lib.nax_set_state(self.h, C.byref(v)), "nax_set_state")
+        if getattr(st.adapt_state, "ss_state", None) is not None:
+            self._set_adapt(st.adapt_state)
 
 
 def _as_model(potential_fn, init_params):

@@ -148,10 +148,18 @@ class MCMC:
         size = (upper - lower) // self.thinning if size is None else size
         i0 = int(eng.get_state().i.max())
         start_idx = i0 + lower + (upper - lower) % self.thinning
-        # HMCECS keeps the inner kernel's state under hmc_state
-        # (HMCECSState, numpyro/infer/hmc_gibbs.py:495-500)
-        ren = {f: f[len("hmc_state."):] for f in collect_fields
-               if f.startswith("hmc_state.") and getattr(self.sampler, "_gibbs_sites", None)}
+        # HMCECS keeps the inner kernel's state under hmc_state and its own
+        # Gibbs acceptance probability as accept_prob (HMCECSState,
+        # numpyro/infer/hmc_gibbs.py:490-491, 660, 681); no other field exists
+        ren = {}
+        if getattr(self.sampler, "_gibbs_sites", None):
+            for f in collect_fields:
+                if f.startswith("hmc_state."):
+                    ren[f] = f[len("hmc_state."):]
+                elif f == "accept_prob":
+                    ren[f] = "_gibbs_accept_prob"
+                elif f != self._sample_field:
+                    raise AttributeError(f"'HMCECSState' object has no attribute {f!r}")
         dev_fields = tuple(ren.get(f, f) for f in collect_fields if f != "z") + ("z",)
         out = eng.run(i0 + upper, collect_fields=dev_fields, start_idx=start_idx,
                       thinning=self.thinning, size=size)

@@ -81,3 +81,39 @@ def make_hier(G=1024, seed=4, nmin=50, nmax=5000, dmin=4, dmax=64):
         Xs.append(X)
         ys.append(y)
     return Xs, ys
+
+
+def laplace(X, y, prior_scale=1.0, iters=12, chunk=1 << 18):
+    """Mode and diagonal covariance of the Laplace approximation of the
+    logistic posterior: Newton iterations on U(b) = -sum log Bernoulli(y |
+    sigmoid(X b)) + |b|^2 / (2 s^2), the Hessian X^T diag(w) X + I / s^2
+    accumulated in float64 over row chunks.  Returns (mode, diag(H^-1)) as
+    float64 tensors on X's device."""
+    N, D = X.shape
+    b = torch.zeros(D, dtype=torch.float64, device=X.device)
+    eye = torch.eye(D, dtype=torch.float64, device=X.device) / prior_scale ** 2
+    for _ in range(iters):
+        g = b / prior_scale ** 2
+        H = eye.clone()
+        for r0 in range(0, N, chunk):
+            xc = X[r0:r0 + chunk].to(torch.float64)
+            p = torch.sigmoid(xc @ b)
+            g += xc.T @ (p - y[r0:r0 + chunk].to(torch.float64))
+            H += xc.T @ (xc * (p * (1 - p))[:, None])
+        step = torch.linalg.solve(H, g)
+        b = b - step
+        if float(step.abs().max()) < 1e-10:
+            break
+    return b, torch.diagonal(torch.linalg.inv(H)).clone()
+
+
+def laplace_init(X, y, C, seed=0):
+    """init_params (mode + sd * uniform(-2, 2) per coordinate and chain) and
+    the initial inverse_mass_matrix (Laplace variances), both float32."""
+    import numpy as np
+    mode, var = laplace(X, y)
+    sd = var.sqrt()
+    u = torch.as_tensor(np.random.default_rng(seed).uniform(-2.0, 2.0, (C, X.shape[1])),
+                        device=X.device)
+    z0 = (mode[None, :] + sd[None, :] * u).to(torch.float32).contiguous()
+    return z0, var.to(torch.float32).contiguous()

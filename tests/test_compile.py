@@ -233,12 +233,25 @@ def test_logistic_with_intercept():
         b = npa.sample("b", dist.Normal(0.0, 2.0).expand([4]).to_event(1))
         npa.sample("y", dist.Bernoulli(logits=b @ X.T + a), obs=y)
 
+    # ravel_pytree flattens the latent dict in sorted site order: [a, b]
     kind, p, _ = match_model(model, X, y)
     assert kind == "logistic" and p["prior_scale"] == 2.0
-    assert p["sites"] == (("b", 4, None), ("a", 1, None))
-    assert p["X"].shape == (120, 5) and bool((p["X"][:, 4] == 1).all())
+    assert p["sites"] == (("a", 1, None), ("b", 4, None))
+    assert p["X"].shape == (120, 5) and bool((p["X"][:, 0] == 1).all())
     z = np.random.default_rng(13).normal(size=5)
     np.testing.assert_allclose(_interpreted_potential(model, (X, y), p["sites"], z),
+                               om.logistic_potential(z, p["X"].numpy(), y.numpy(), 2.0),
+                               rtol=1e-10)
+
+    def model2(X, y):  # an intercept whose name sorts last
+        b = npa.sample("b", dist.Normal(0.0, 2.0).expand([4]).to_event(1))
+        c = npa.sample("c", dist.Normal(0.0, 2.0))
+        npa.sample("y", dist.Bernoulli(logits=X @ b + c), obs=y)
+
+    kind, p, _ = match_model(model2, X, y)
+    assert p["sites"] == (("b", 4, None), ("c", 1, None))
+    assert bool((p["X"][:, 4] == 1).all())
+    np.testing.assert_allclose(_interpreted_potential(model2, (X, y), p["sites"], z),
                                om.logistic_potential(z, p["X"].numpy(), y.numpy(), 2.0),
                                rtol=1e-10)
 

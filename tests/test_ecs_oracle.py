@@ -180,3 +180,29 @@ def test_gibbs_step_accepts_by_the_uniform():
     assert ap == 1.0 and g2 is not None
     np.testing.assert_allclose(pe2, pe, rtol=1e-12)
     np.testing.assert_array_equal(k, split(PRNGKey(3))[0])
+
+
+def test_gibbs_step_rejects_a_nan_proposal():
+    # hmc_gibbs.py:660-661: clip(exp(pe - pe_new), None, 1) is NaN for a NaN
+    # proposal energy, and bernoulli(key, NaN) = uniform < NaN is false
+    import warnings
+    rng = np.random.default_rng(0)
+    N, D, m = 50, 3, 10
+    X = rng.normal(size=(N, D))
+    X[:25, 0] = np.nan
+    y = (rng.uniform(size=N) < 0.5).astype(np.float64)
+    idx = rng.integers(25, N, m)
+    beta = np.zeros(D)
+    pe = ecs.potential(beta, idx, X, y, N)
+    assert np.isfinite(pe)
+    seen = 0
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        for s in range(20):
+            key = np.array([0, s], np.uint32)
+            k, idx_new, pe_new, g, ap = ecs.gibbs_step(key, idx, beta, pe, X, y, N, 1)
+            if np.isnan(ap):
+                seen += 1
+                assert g is None and pe_new == pe
+                np.testing.assert_array_equal(idx_new, idx)
+    assert seen >= 10

@@ -152,12 +152,16 @@ def test_compiled_intercept_matches_augmented_builtin():
         a = npa.sample("a", dist.Normal(0.0, 1.5))
         npa.sample("y", dist.Bernoulli(logits=X @ b + a), obs=y)
 
-    m = MCMC(NUTS(model), num_warmup=100, num_samples=100, num_chains=2)
+    # the flat latent is [a, b] (sorted site names, as ravel_pytree), so a
+    # flat inverse_mass_matrix applies its first entry to the intercept
+    imm = torch.tensor([0.5, 1.0, 2.0, 3.0])
+    m = MCMC(NUTS(model, inverse_mass_matrix=imm, adapt_mass_matrix=False), num_warmup=100,
+             num_samples=100, num_chains=2)
     m.run(random.PRNGKey(8), X, y)
-    Xa = torch.cat([X, torch.ones(X.shape[0], 1)], 1)
-    r = MCMC(NUTS(LogisticRegression(Xa, y, prior_scale=1.5)), num_warmup=100, num_samples=100,
-             num_chains=2)
+    Xa = torch.cat([torch.ones(X.shape[0], 1), X], 1)
+    r = MCMC(NUTS(LogisticRegression(Xa, y, prior_scale=1.5), inverse_mass_matrix=imm,
+                  adapt_mass_matrix=False), num_warmup=100, num_samples=100, num_chains=2)
     r.run(random.PRNGKey(8))
     s, t = m.get_samples(), r.get_samples()["coefs"]
     assert set(s) == {"a", "b"}
-    assert torch.equal(s["b"], t[:, :3]) and torch.equal(s["a"], t[:, 3])
+    assert torch.equal(s["a"], t[:, 0]) and torch.equal(s["b"], t[:, 1:])

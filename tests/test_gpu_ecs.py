@@ -123,6 +123,45 @@ def test_gibbs_step_vs_oracle(degree, nb):
     assert n_acc + n_rej >= C - 3
 
 
+def test_gibbs_step_rejects_nan_proposals():
+    # hmc_gibbs.py:660-661: accept_prob = clip(exp(pe - pe_new), None, 1) keeps
+    # a NaN, and bernoulli(key, NaN) rejects.  Rows 0..9 of X hold a NaN; the
+    # chains start on subsamples without them, so a proposal that draws one
+    # has a NaN potential and must be rejected (the accept_prob stays NaN).
+    from numpyro_amd import _lib
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.infer.hmc import Engine
+    N, D, m, C = 400, 4, 20, 32
+    X, y, beta = _data(N, D, 8)
+    X[:10, 1] = np.nan
+    mdl, idx, keys = _model(X, y, m, 0, None, C, seed=9, nb=1)
+    idx = np.random.default_rng(10).integers(10, N, (C, m)).astype(np.int32)
+    mdl.idx = torch.as_tensor(idx).cuda()
+    eng = Engine(mdl, C, "NUTS", 10)
+    zc = beta + np.random.default_rng(11).normal(0, 0.1, (C, D))
+    eng.init(torch.tensor(zc, dtype=torch.float32),
+             np.random.default_rng(12).integers(0, 2 ** 32, (C, 2), dtype=np.uint64)
+             .astype(np.uint32))
+    pe0 = np.asarray(eng.get_state().potential_energy, dtype=np.float64)
+    assert np.isfinite(pe0).all()
+    acc = torch.zeros(C, dtype=torch.float32, device="cuda")
+    _lib.check(eng.lib.nax_ecs_gibbs(eng.h, _lib.ptr(acc)), "gibbs")
+    idx1 = mdl.idx.cpu().numpy()
+    pe1 = np.asarray(eng.get_state().potential_energy, dtype=np.float64)
+    acc = acc.cpu().numpy()
+    z32 = torch.tensor(zc, dtype=torch.float32).numpy().astype(np.float64)
+    n_nan = 0
+    for c in range(C):
+        k, idx_o, pe_o, g_o, ap = ecs.gibbs_step(keys[c], idx[c], z32[c], pe0[c], X, y, N, 1,
+                                                 None, 1.3, nrandom.PARTITIONABLE)
+        if np.isnan(ap):
+            n_nan += 1
+            assert np.isnan(acc[c]) and g_o is None
+            np.testing.assert_array_equal(idx1[c], idx[c])
+            assert pe1[c] == pe0[c]
+    assert n_nan >= 3, n_nan
+
+
 def _covtype(data, labels, subsample_size=None):
     # examples/covtype.py:66-71
     import numpyro_amd as npa
@@ -237,3 +276,28 @@ def test_estimated_potential_tracks_the_full_potential():
     full = potential_energy(LogisticRegression(Xt, yt), z).double()
     r = torch.exp(full - pes)
     assert float(r.var()) < 1.0, (float(r.mean()), float(r.var()))
+
+
+def test_hmcecs_extra_fields_accept_prob_is_the_gibbs_acceptance():
+    # HMCECSState (hmc_gibbs.py:490-491, 681): accept_prob is the Gibbs
+    # block acceptance of each iteration, hmc_state.accept_prob the inner
+    # kernel's; a bare HMCState field does not exist on HMCECSState
+    from numpyro_amd import random
+    from numpyro_amd.infer import HMCECS, MCMC, NUTS
+    X, y, beta = _data(3000, 3, 21)
+    Xt, yt = torch.as_tensor(X), torch.as_tensor(y)
+    prox = HMCECS.taylor_proxy({"coefs": torch.as_tensor(beta, dtype=torch.float32)})
+    kern = HMCECS(NUTS(_covtype), num_blocks=10, proxy=prox)
+    mcmc = MCMC(kern, num_warmup=50, num_samples=40, num_chains=3)
+    mcmc.run(random.PRNGKey(5), Xt, yt, 200,
+             extra_fields=("accept_prob", "hmc_state.accept_prob"))
+    ex = mcmc.get_extra_fields(group_by_chain=True)
+    ga, ha = ex["accept_prob"], ex["hmc_state.accept_prob"]
+    assert ga.shape == (3, 40) and ha.shape == (3, 40)
+    assert bool(((ga >= 0) & (ga <= 1)).all())
+    assert torch.equal(ga[:, -1].cpu(), kern.gibbs_accept_prob.cpu())
+    assert not torch.equal(ga.cpu(), ha.cpu())
+    mcmc2 = MCMC(HMCECS(NUTS(_covtype), num_blocks=10, proxy=prox), num_warmup=5,
+                 num_samples=5)
+    with pytest.raises(AttributeError, match="HMCECSState"):
+        mcmc2.run(random.PRNGKey(6), Xt, yt, 200, extra_fields=("diverging",))
