@@ -242,6 +242,24 @@ int nax_buffers(nax_sampler* h, float** d_z_request, double** d_potential_energy
  * the dense-layout form of nax_buffers. */
 int nax_exchange(nax_sampler* h, float* d_z_out, const double* d_pe_in, const float* d_grad_in);
 
+/* The bijection that unconstrains the model's positive latent sites
+ * (potential_energy's _unconstrain_reparam, numpyro/infer/util.py:300-330):
+ * kind 0 = ExpTransform, biject_to(constraints.positive) (transforms.py:
+ * 606-652, 2115-2118, the default); kind 1 = SoftplusTransform,
+ * biject_to(constraints.softplus_positive) (transforms.py:1156-1175,
+ * 2213-2215).  Applies to sigma and nu of the stochastic-volatility model
+ * and tau of the hierarchical model; NAX_EINVAL for a model without
+ * positive sites, NAX_ESTATE after nax_init. */
+int nax_set_positive_transform(nax_sampler* h, int32_t kind);
+
+/* The launch shapes the model's autotuner picked (waves per chain of a
+ * persistent launch, row splits / kernel form of the logistic kernels, tile
+ * rows of the hierarchical kernel, with the timings it compared), as text
+ * into buf (at most len bytes with the terminating zero; empty before the
+ * first tuned launch).  No counterpart in the reference, which compiles one
+ * XLA program per model. */
+int nax_tuning(nax_sampler* h, char* buf, int32_t len);
+
 /* HMCState / post_warmup_state access (mcmc.py:558-587): copies the
  * per-chain state out of / into the sampler. */
 int nax_get_state(nax_sampler* h, const nax_state_view* v);

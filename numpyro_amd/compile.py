@@ -34,7 +34,8 @@ import numpy as np
 import torch
 
 from . import distributions as dist
-from .models import HierarchicalLogistic, LogisticRegression, Model, StochasticVolatility
+from .models import (HierarchicalLogistic, LogisticRegression, Model, StochasticVolatility,
+                     constrain)
 from .primitives import Messenger, substitute, trace
 
 # ---- symbolic values ----------------------------------------------------------
@@ -586,7 +587,7 @@ class CompiledModel(Model):
             v = z[..., o:o + size]
             if size == 1:
                 v = v[..., 0]
-            out[name] = torch.exp(v) if tf == "exp" else v
+            out[name] = constrain(v, tf)
             o += size
         return out
 

@@ -48,6 +48,7 @@ struct HierData {
   const int* order;  // block -> model, heaviest first
   int K;             // chains per model
   int Dp;            // stride of a chain's latent vector
+  int softplus;      // tau by SoftplusTransform instead of ExpTransform
 };
 
 // Row stride of a staged X tile: 68 floats keeps both access patterns free
@@ -100,7 +101,8 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   const float* z = zreq + (size_t)c * ldz;
   float* g = gout + (size_t)c * ldg;
   const float mu = z[D];
-  const float tau = expf(z[D + 1]);
+  const PosTf tt = positive_tf((double)z[D + 1], hd.softplus != 0);
+  const float tau = hd.softplus ? (float)tt.x : expf(z[D + 1]);
   const float eta = lane < D ? z[lane] : 0.0f;
   sh.beta[w][lane] = lane < D ? mu + tau * eta : 0.0f;
   // columns [D, D4) of both buffers stay zero for the whole evaluation
@@ -201,12 +203,15 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   const double t2 = (double)tau * tau;
   if (lane == 0) {
     g[D] = (float)((double)mu + sum_acc);
-    g[D + 1] = (float)((double)tau * eta_acc + 2.0 * t2 / (1.0 + t2) - 1.0);
+    // softplus: dtau/du (eta.acc + 2 tau / (1 + tau^2)) - d log|J| / du
+    g[D + 1] = hd.softplus
+        ? (float)(tt.dx * (eta_acc + 2.0 * (double)tau / (1.0 + t2)) - tt.dladj)
+        : (float)((double)tau * eta_acc + 2.0 * t2 / (1.0 + t2) - 1.0);
   }
   const double U = U_lik + 0.5 * eta2 + D * kLog2PiHalf          // eta ~ N(0,1)
                    + 0.5 * (double)mu * mu + kLog2PiHalf          // mu ~ N(0,1)
                    - (kLog2 - kLogPi_ - log1p(t2))                // HalfCauchy(1)
-                   - (double)z[D + 1];                            // log|J| of exp
+                   - (hd.softplus ? tt.ladj : (double)z[D + 1]);  // log|J|
   return U;
 }
 
@@ -284,6 +289,10 @@ struct HierPotential : Potential {
   }
   int dim() const override { return maxd + 2; }
   const char* name() const override { return "hier"; }
+  bool set_positive_transform(int kind) override {
+    hd.softplus = kind == 1;
+    return true;
+  }
   bool persistent() const override { return true; }
   const int* chain_dims() const override { return cdims; }
   double alg_bytes(int) const override { return bytes; }
@@ -382,7 +391,7 @@ Potential* make_hier(const float* x, const float* y, const int64_t* d_row_off,
     delete p;
     return nullptr;
   }
-  p->hd = HierData{x, y, p->row_off, p->elt_off, p->dims, p->order, K, Dp};
+  p->hd = HierData{x, y, p->row_off, p->elt_off, p->dims, p->order, K, Dp, 0};
   return p;
 }
 

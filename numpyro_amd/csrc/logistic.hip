@@ -59,7 +59,7 @@ logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int6
   for (int j = 0; j < DCH; ++j)
 #pragma unroll
     for (int k = 0; k < CK; ++k) acc[j][k] = 0.0f;
-  double uacc = 0.0;  // per (row, chain) pair owned in the L phase
+  CompSum uacc;  // per (row, chain) pair owned in the L phase, compensated
   constexpr int PAIRS = BN * BC / 256;
   static_assert(PAIRS >= 1 && (BN * BC) % 256 == 0, "tile");
 
@@ -88,7 +88,7 @@ logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int6
       if (n < N && c0 + c < C) {
         float yv = y[n];
         rv = sigmoidf_(l) - yv;
-        uacc += (double)(softplusf_(l) - yv * l);
+        uacc.add((double)(softplusf_(l) - yv * l));
       }
       Rs[r * BC + c] = rv;
     }
@@ -117,12 +117,12 @@ logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int6
   }
   // energy partials: pairs q = t + 256 p have chain q / BN; with PAIRS == 1
   // threads [c*BN, (c+1)*BN) own chain c.
-  Ud[t] = uacc;
+  Ud[t] = uacc.value();
   __syncthreads();
   if (t < BC && c0 + t < C) {
-    double s = 0.0;
-    for (int q = t * BN; q < (t + 1) * BN; ++q) s += Ud[q % 256];
-    wsu[(size_t)rb * C + c0 + t] = s;
+    CompSum s;
+    for (int q = t * BN; q < (t + 1) * BN; ++q) s.add(Ud[q % 256]);
+    wsu[(size_t)rb * C + c0 + t] = s.value();
   }
 }
 
@@ -164,9 +164,11 @@ logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int 
     __syncthreads();
   }
   if (t == 0) {
-    double u = 0.0;
-    for (int r = 0; r < RB; ++r) u += wsu[(size_t)r * ldw + ci];
-    pe[c] = u + red[0] + u_offset;
+    CompSum u;  // the RB row-split partials, compensated
+    for (int r = 0; r < RB; ++r) u.add(wsu[(size_t)r * ldw + ci]);
+    u.add(red[0]);
+    u.add(u_offset);
+    pe[c] = u.value();
   }
 }
 

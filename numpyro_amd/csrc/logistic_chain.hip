@@ -31,7 +31,9 @@ struct LogisticChainModel {
   float inv_s2;
   double log_norm;  // d * log(sqrt(2 pi) s)
 
-  __device__ double value_and_grad(const float* z, float* g, int lane) const {
+  template <class G>
+  __device__ double value_and_grad(const G& grp, const float* z, float* g) const {
+    const int lane = grp.lane & (NAX_WAVE - 1);
     float zr[DMAX], gp[DMAX];
 #pragma unroll
     for (int k = 0; k < DMAX; ++k) {
@@ -39,7 +41,7 @@ struct LogisticChainModel {
       gp[k] = 0.0f;
     }
     double u = 0.0;
-    for (int64_t row = lane; row < n; row += NAX_WAVE) {
+    for (int64_t row = grp.lane; row < n; row += G::NT) {
       const float* xr = x + row * d;
       float xv[DMAX];
       float l0 = 0.0f, l1 = 0.0f;
@@ -84,8 +86,22 @@ struct LogisticChainModel {
     constexpr int shift = DMAX == 64 ? 0 : (DMAX == 32 ? 1 : -1);
     static_assert(shift >= 0, "DMAX is 32 or 64");
     const int k = lane >> shift;
-    if ((lane & ((1 << shift) - 1)) == 0 && k < d) g[k] = gp[0] + z[k] * inv_s2;
-    return wave_sum(u) + pr + log_norm;
+    if constexpr (G::WPC == 1) {
+      if ((lane & ((1 << shift) - 1)) == 0 && k < d) g[k] = gp[0] + z[k] * inv_s2;
+    } else {
+      // the waves' partial gradients meet in LDS, summed in wave order
+      const int w = grp.lane / NAX_WAVE;
+      __syncthreads();
+      if ((lane & ((1 << shift) - 1)) == 0 && k < d) grp.redf[w * NAX_WAVE + k] = gp[0];
+      __syncthreads();
+      if (grp.lane < d) {
+        float t = 0.0f;
+#pragma unroll
+        for (int i = 0; i < G::WPC; ++i) t += grp.redf[i * NAX_WAVE + grp.lane];
+        g[grp.lane] = t + z[grp.lane] * inv_s2;
+      }
+    }
+    return chain_sum(grp, u) + pr + log_norm;
   }
 };
 
@@ -110,18 +126,24 @@ struct LogisticChainPotential : Potential {
   hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
                             int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
                             float* gout, double* peout, int64_t* gevals) override {
+    // one chain per workgroup; the waves per chain (rows of X split over
+    // 64 wpc threads) are picked on the first launch by timing a few
+    // iterations of each candidate from a scratch copy of the state
+    using Mdl = LogisticChainModel<DMAX>;
+    if (wpc == 0 || tuned_chains != cfg.C) {
+      std::string log;
+      wpc = tune_chain_groups<Mdl, false>(s, m, cfg, st, vec, wel, zreq, gout, peout, 4, &log);
+      tuned_chains = cfg.C;
+      tuning = "logistic_chain: " + std::to_string(wpc) + " waves per chain (ms for 4 "
+               "iterations:" + log + ")";
+    }
     mark_begin(s);
-    if (cfg.dense)
-      hipLaunchKernelGGL((persistent_kernel<LogisticChainModel<DMAX>, true, 1>), dim3(cfg.C),
-                         dim3(64), 0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout,
-                         gevals, 0);
-    else
-      hipLaunchKernelGGL((persistent_kernel<LogisticChainModel<DMAX>, false, 1>), dim3(cfg.C),
-                         dim3(64), 0, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout,
-                         gevals, 0);
+    hipError_t e = launch_chain_groups<Mdl, false>(wpc, s, m, cfg, col, target, st, vec, wel,
+                                                   zreq, gout, peout, gevals);
     mark_end(s);
-    return hipGetLastError();
+    return e;
   }
+  int wpc = 0, tuned_chains = 0;
 };
 
 // d <= 64; x and y are the caller's device arrays (kept, not copied).

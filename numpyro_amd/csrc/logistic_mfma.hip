@@ -211,7 +211,7 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   for (int dt = 0; dt < DT; ++dt)
 #pragma unroll
     for (int v = 0; v < 16; ++v) g[dt][v] = 0.0f;
-  double u = 0.0;
+  CompSum u;  // fp64 energy over this lane's tiles, compensated
 
   // ---- tile staging: global -> registers (prefetch) -> LDS ----
   uint4 pf[PF];
@@ -272,7 +272,7 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
         us += uu;
       }
     }
-    u += (double)us;
+    u.add((double)us);
     // GEMM2: G^T += R^T X_tile
     if constexpr (BF) {
 #pragma unroll
@@ -317,7 +317,11 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
       if (ci < n_act && d < D) wsg[((size_t)rb * ldw + ci) * D + d] = g[dt][v];
     }
   }
-  const double ut = u + __shfl_xor(u, 32, 64);
+  CompSum uo;
+  uo.s = __shfl_xor(u.s, 32, 64);
+  uo.c = __shfl_xor(u.c, 32, 64);
+  u.add(uo);
+  const double ut = u.value();
   if (h == 0 && cvalid) wsu[(size_t)rb * ldw + c0 + cl] = ut;
 }
 
@@ -433,7 +437,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
   for (int dt = 0; dt < DT; ++dt)
 #pragma unroll
     for (int v = 0; v < 16; ++v) g[dt][v] = 0.0f;
-  double u = 0.0;
+  CompSum u;  // fp64 energy over this lane's tiles, compensated
 
   const int64_t n = rb < ntile ? (ntile - 1 - rb) / RB + 1 : 0;
   // Tile i is clamped to the block's last tile, so every load is in bounds
@@ -565,7 +569,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       if (v < 8) r0[v] = (typename std::remove_reference<decltype(r0[0])>::type)r;
       else r1[v - 8] = (typename std::remove_reference<decltype(r1[0])>::type)r;
     }
-    u += (double)us;
+    u.add((double)us);
   };
   // steady-state step: epilogue(i) || GEMM1(i+1) || GEMM2(i-1), interleaved
   // element by element in source order.
@@ -597,7 +601,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       if (e < 8) r0[e] = (typename std::remove_reference<decltype(r0[0])>::type)r;
       else r1[e - 8] = (typename std::remove_reference<decltype(r1[0])>::type)r;
     }
-    u += (double)us;
+    u.add((double)us);
     return Ln;
   };
   auto fused_step256 = [&](int64_t i, const f32x16& Lc, rfrag rp0, rfrag rp1, rfrag& r0,
@@ -642,7 +646,7 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-    u += (double)us;
+    u.add((double)us);
     return Ln;
   };
 
@@ -700,7 +704,11 @@ logit_bf16_pipe(const bf16x8* __restrict__ A1, const f16x8* __restrict__ B2,
       if (ci < n_act && d < D) wsg[((size_t)rb * ldw + ci) * D + d] = g[dt][v];
     }
   }
-  const double ut = u + __shfl_xor(u, 32, 64);
+  CompSum uo;
+  uo.s = __shfl_xor(u.s, 32, 64);
+  uo.c = __shfl_xor(u.c, 32, 64);
+  u.add(uo);
+  const double ut = u.value();
   if (h == 0 && cvalid) wsu[(size_t)rb * ldw + c0 + cl] = ut;
 }
 

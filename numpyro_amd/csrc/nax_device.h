@@ -132,6 +132,44 @@ __device__ __forceinline__ float uniform_vec(Key k, uint32_t i, uint32_t n, bool
   return fmaxf(lo, f * (hi - lo) + lo);
 }
 
+// Positive latent sites in unconstrained space (potential_energy's
+// _unconstrain_reparam, numpyro/infer/util.py:300-330): biject_to(positive)
+// is ExpTransform (transforms.py:606-652, 2115-2118; x = e^u, log|J| = u),
+// biject_to(softplus_positive) is SoftplusTransform (transforms.py:1156-1175,
+// 2213-2215; x = softplus(u) = log1p(e^u), log|J| = -softplus(-u)).  For u:
+// x, dx/du, log|dx/du| and its derivative d log|dx/du| / du.
+struct PosTf {
+  double x, dx, ladj, dladj;
+};
+__device__ __forceinline__ PosTf positive_tf(double u, bool softplus) {
+  if (!softplus) {
+    const double x = exp(u);
+    return PosTf{x, x, u, 1.0};
+  }
+  const double e = exp(-fabs(u));
+  const double sp = fmax(u, 0.0) + log1p(e);            // softplus(u), no overflow
+  const double sg = u >= 0.0 ? 1.0 / (1.0 + e) : e / (1.0 + e);   // sigmoid(u)
+  const double sgn = u >= 0.0 ? e / (1.0 + e) : 1.0 / (1.0 + e);  // sigmoid(-u)
+  return PosTf{sp, sg, u - sp, sgn};                    // -softplus(-u) = u - softplus(u)
+}
+
+// Neumaier's compensated summation (improved Kahan-Babuska, Neumaier 1974):
+// `s + c` carries the running sum with the rounding error of every addition
+// kept in `c`.  Used for the fp64 energy sums across tiles and partials.
+struct CompSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ void add(const CompSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
 // random.bernoulli(key, p) for a scalar p: uniform(key, ()) < p.
 __device__ __forceinline__ bool bernoulli(Key k, double p, bool part) {
   return (double)bits_to_unit(bits_scalar(k, part)) < p;

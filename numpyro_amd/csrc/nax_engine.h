@@ -1,7 +1,12 @@
 // The NUTS / HMC per-chain state machine as inline device functions, shared
 // by the tick kernel (sampler.hip) and the persistent per-model kernels.
 //
-// One wavefront owns one chain.  advance_chain() runs the chain's control flow
+// One wavefront owns one chain, or (persistent launches of latency-bound
+// models) a group of WPC wavefronts of one workgroup does: vector passes
+// then stride over the group's 64 WPC threads, reductions and scans combine
+// the waves through LDS, and every wave runs the scalar control flow on its
+// own copy of the chain state (the copies stay identical: every reduction
+// hands all threads the same value).  advance_chain() runs the chain's control flow
 // -- sample_kernel (numpyro/infer/hmc.py:459-530), build_tree
 // (numpyro/infer/hmc_util.py:1088-1180) with the iterative subtree, the
 // checkpointed U-turn checks and the uniform/biased transitions
@@ -12,6 +17,12 @@
 // previous call.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
 
 #include "nax_device.h"
 #include "nax_internal.h"
@@ -44,17 +55,32 @@ enum {
 __device__ __forceinline__ Key K(const uint32_t* k) { return Key{k[0], k[1]}; }
 __device__ __forceinline__ void setK(uint32_t* d, Key k) { d[0] = k.k0; d[1] = k.k1; }
 
+// A translation unit whose chains have long latent vectors (thousands of
+// entries per lane group) defines NAX_FOR_D_UNROLL before including this
+// header, so that several of a lane's loads are in flight at once; the
+// default keeps register pressure low for short vectors.
+#ifndef NAX_FOR_D_UNROLL
+#define NAX_FOR_D_UNROLL 1
+#endif
+
 // LDSV: the first nl slots may live in LDS (one-wavefront persistent
 // launches of long-vector models).  Only that context pays for the run-time
 // slot test and for generic (flat) vector pointers; the others address the
 // global vector block directly.
-template <bool LDSV>
+template <bool LDSV, int WPC_ = 1>
 struct ChainCtxT {
+  static constexpr int WPC = WPC_;          // wavefronts working on the chain
+  static constexpr int NT = NAX_WAVE * WPC_;  // threads working on the chain
+  // FOR_D's unroll: the translation unit's factor for one wave, divided by
+  // the waves (each thread walks WPC times fewer elements)
+  static constexpr int UNROLL = NAX_FOR_D_UNROLL / WPC_ > 0 ? NAX_FOR_D_UNROLL / WPC_ : 1;
+  double* red;     // LDS reduction slots of the group (kRedSlots * WPC), WPC > 1
+  float* redf;     // LDS [WPC][64] fp32 exchange of the group, WPC > 1
   float* vb;       // chain's vector block [nvec][Dp]
   float* lb;       // LDS copy of slots [0, nl) when LDSV, else unused
   int nl;
   double* wel;     // Welford [2][Dp] (mean, m2) fp64
-  int lane, D, Dp;
+  int lane, D, Dp;  // lane: the thread's index in the chain's group, 0..NT-1
   // dense mass matrix ([Dp][Dp] row-major blocks, nullptr when diagonal):
   // mi[k][d] = M^-1[d][k], ms[k][d] = mass_matrix_sqrt[d][k], m2 fp64
   float* mi;
@@ -67,19 +93,90 @@ struct ChainCtxT {
 };
 using ChainCtx = ChainCtxT<false>;
 
-// A translation unit whose chains have long latent vectors (thousands of
-// entries per lane group) defines NAX_FOR_D_UNROLL before including this
-// header, so that several of a lane's loads are in flight at once; the
-// default keeps register pressure low for short vectors.
-#ifndef NAX_FOR_D_UNROLL
-#define NAX_FOR_D_UNROLL 1
-#endif
 #define NAX_PRAGMA_(x) _Pragma(#x)
 #define NAX_PRAGMA(x) NAX_PRAGMA_(x)
-#define FOR_D(d) NAX_PRAGMA(unroll NAX_FOR_D_UNROLL) for (int d = cx.lane; d < cx.D; d += NAX_WAVE)
+#define NAX_CTX_T std::remove_cv_t<std::remove_reference_t<decltype(cx)>>
+#define FOR_D(d) \
+  NAX_PRAGMA(unroll(NAX_CTX_T::UNROLL)) for (int d = cx.lane; d < cx.D; d += cx.NT)
 
 // Makes this wavefront's earlier global writes visible to its other lanes.
 __device__ __forceinline__ void wave_sync_mem() { __threadfence_block(); }
+
+// ---- chain-group primitives (one wave, or WPC waves of one workgroup) -------------
+constexpr int kRedSlots = 8;  // values one chain_sum_n can combine
+
+// Makes the group's earlier writes (global and LDS) visible to all its threads.
+template <class G>
+__device__ __forceinline__ void chain_sync(const G& g) {
+  if constexpr (G::WPC == 1) __threadfence_block();
+  else __syncthreads();
+}
+
+// Sums of N values over the group's threads; every thread receives the same
+// sums (the waves' partials are added in wave order).
+template <int N, class G>
+__device__ __forceinline__ void chain_sum_n(const G& g, double (&v)[N]) {
+  static_assert(N <= kRedSlots, "chain_sum_n: too many values");
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+  if constexpr (G::WPC > 1) {
+    const int w = g.lane / NAX_WAVE;
+    __syncthreads();  // earlier readers of the slots are done
+    if ((g.lane & (NAX_WAVE - 1)) == 0)
+#pragma unroll
+      for (int i = 0; i < N; ++i) g.red[i * G::WPC + w] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < G::WPC; ++k) t += g.red[i * G::WPC + k];
+      v[i] = t;
+    }
+  }
+}
+
+template <class G>
+__device__ __forceinline__ double chain_sum(const G& g, double x) {
+  double v[1] = {x};
+  chain_sum_n(g, v);
+  return v[0];
+}
+
+// Inclusive prefix sum over the group's threads in lane order.
+__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < NAX_WAVE; o <<= 1) {
+    const double t = __shfl_up(v, o, NAX_WAVE);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+template <class G>
+__device__ __forceinline__ double chain_inclusive_scan(const G& g, double v) {
+  v = wave_inclusive_scan(v, g.lane & (NAX_WAVE - 1));
+  if constexpr (G::WPC > 1) {
+    const int w = g.lane / NAX_WAVE;
+    __syncthreads();
+    if ((g.lane & (NAX_WAVE - 1)) == NAX_WAVE - 1) g.red[w] = v;
+    __syncthreads();
+    double off = 0.0;
+    for (int k = 0; k < w; ++k) off += g.red[k];
+    v += off;
+  }
+  return v;
+}
+
+// The group a standalone (non-persistent) evaluation of one chain runs on:
+// one wavefront.
+struct WaveGroup {
+  static constexpr int WPC = 1;
+  static constexpr int NT = NAX_WAVE;
+  double* red = nullptr;
+  float* redf = nullptr;
+  int lane;
+};
 
 // (A x)[d] for a matrix stored transposed (at[k][d] = A[d][k]): lane d walks
 // column d of `at` (coalesced across lanes) against the broadcast x[k].
@@ -109,13 +206,13 @@ __device__ __forceinline__ double kinetic(const CX& cx, int r_slot) {
   const float* r = cx.v(r_slot);
   double acc = 0.0;
   if (cx.mi) {
-    wave_sync_mem();
+    chain_sync(cx);
     FOR_D(d) acc += (double)dense_mv(cx, cx.mi, r, d) * (double)r[d];
   } else {
     const float* m = cx.v(V_IMM);
     FOR_D(d) { double rv = r[d]; acc += (double)m[d] * rv * rv; }
   }
-  return 0.5 * wave_sum(acc);
+  return 0.5 * chain_sum(cx, acc);
 }
 
 // _is_turning (hmc_util.py:710-746) with r_sum given as rs - rs_ck + r_ck when
@@ -125,7 +222,7 @@ __device__ __forceinline__ bool is_turning(const CX& cx, const float* rl, const 
                            const float* rs, const float* rs_ck, const float* r_ck) {
   const float* m = cx.v(V_IMM);
   double a = 0.0, b = 0.0;
-  if (cx.mi) wave_sync_mem();
+  if (cx.mi) chain_sync(cx);
   FOR_D(d) {
     float s = rs[d];
     if (rs_ck) s = s - rs_ck[d] + r_ck[d];
@@ -137,9 +234,9 @@ __device__ __forceinline__ bool is_turning(const CX& cx, const float* rl, const 
     a += (double)vl * c;
     b += (double)vr * c;
   }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  return (a <= 0.0) || (b <= 0.0);
+  double ab[2] = {a, b};
+  chain_sum_n(cx, ab);
+  return (ab[0] <= 0.0) || (ab[1] <= 0.0);
 }
 
 // momentum_generator (hmc.py:92-110), diagonal mass: s * normal(key, (D,)) with
@@ -153,10 +250,10 @@ __device__ __forceinline__ void draw_momentum(const CX& cx, int dst, Key k, bool
   if (cx.mi) {
     float* eps = cx.v(V_TMP);
     FOR_D(d) eps[d] = normal_vec(k, (uint32_t)d, (uint32_t)cx.D, part);
-    wave_sync_mem();
+    chain_sync(cx);
     const float* at = (scale_slot == V_IMM) ? cx.mi : cx.ms;
     FOR_D(d) r[d] = dense_mv(cx, at, eps, d);
-    wave_sync_mem();
+    chain_sync(cx);
     return;
   }
   const float* s = cx.v(scale_slot);
@@ -175,7 +272,7 @@ __device__ __forceinline__ void leapfrog_half(const CX& cx, int zs, int rs, int 
   const float h = 0.5f * eps;
   if (cx.mi) {
     FOR_D(d) r[d] = r[d] - h * g[d];
-    wave_sync_mem();
+    chain_sync(cx);
     FOR_D(d) {
       const float zn = z[d] + eps * dense_mv(cx, cx.mi, r, d);
       z[d] = zn;
@@ -743,11 +840,16 @@ __device__ __forceinline__ void copy_state(ChainState& dst, const ChainState& sr
 
 // DENSE = false gives the diagonal-mass kernels a context whose dense
 // pointers are compile-time null, so the dense branches fold away.
-template <bool DENSE = true, bool LDSV = false>
-__device__ __forceinline__ ChainCtxT<LDSV> make_ctx(const EngineCfg& cfg, int c, float* vec,
-                                                    double* welf) {
-  ChainCtxT<LDSV> cx;
-  cx.lane = threadIdx.x & (NAX_WAVE - 1);
+// WPC > 1 (a workgroup per chain) is for diagonal mass only: the dense
+// factorisation runs on one wavefront.
+template <bool DENSE = true, bool LDSV = false, int WPC = 1>
+__device__ __forceinline__ ChainCtxT<LDSV, WPC> make_ctx(const EngineCfg& cfg, int c,
+                                                         float* vec, double* welf) {
+  static_assert(WPC == 1 || !DENSE, "a multi-wave chain group needs the diagonal mass");
+  ChainCtxT<LDSV, WPC> cx;
+  cx.red = nullptr;
+  cx.redf = nullptr;
+  cx.lane = WPC == 1 ? (int)(threadIdx.x & (NAX_WAVE - 1)) : (int)threadIdx.x;
   cx.D = cfg.chain_dim ? cfg.chain_dim[c] : cfg.D;
   cx.Dp = cfg.Dp;
   cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
@@ -763,10 +865,13 @@ __device__ __forceinline__ ChainCtxT<LDSV> make_ctx(const EngineCfg& cfg, int c,
 
 // ---- per-chain potentials ------------------------------------------------------
 // A per-chain model M provides
-//   __device__ double value_and_grad(const float* z, float* g, int lane) const
-// run by one wavefront: U(z) (returned on every lane) and dU/dz written to g.
-// z and g may be written by other lanes of the same wave just before the call,
-// so callers fence (workgroup scope) around it.
+//   template <class G> __device__ double value_and_grad(const G& grp,
+//                                                       const float* z, float* g) const
+// run by the chain's group G (one wavefront, WaveGroup, or the WPC waves of a
+// chain context; grp.lane = thread index in the group, G::NT threads, the
+// chain_sum / chain_inclusive_scan primitives): U(z) (returned on every
+// thread) and dU/dz written to g.  z and g may be written by other threads of
+// the group just before the call, so callers synchronise around it.
 
 template <class M>
 __global__ void __launch_bounds__(256)
@@ -774,17 +879,38 @@ perchain_eval_kernel(M m, const float* __restrict__ z, int ldz, int C, double* _
                      float* __restrict__ g, int ldg) {
   const int c = blockIdx.x * (blockDim.x / NAX_WAVE) + threadIdx.x / NAX_WAVE;
   if (c >= C) return;
-  const int lane = threadIdx.x & (NAX_WAVE - 1);
-  const double u = m.value_and_grad(z + (size_t)c * ldz, g + (size_t)c * ldg, lane);
-  if (lane == 0) pe[c] = u;
+  WaveGroup grp;
+  grp.lane = threadIdx.x & (NAX_WAVE - 1);
+  const double u = m.value_and_grad(grp, z + (size_t)c * ldz, g + (size_t)c * ldg);
+  if (grp.lane == 0) pe[c] = u;
+}
+
+// The same evaluation with WPC wavefronts (one workgroup) per chain.
+template <class M, int WPC>
+__global__ void __launch_bounds__(64 * WPC)
+perchain_eval_group_kernel(M m, const float* __restrict__ z, int ldz, int C,
+                           double* __restrict__ pe, float* __restrict__ g, int ldg) {
+  __shared__ double red[kRedSlots * WPC];
+  __shared__ float redf[NAX_WAVE * WPC];
+  const int c = blockIdx.x;
+  if (c >= C) return;
+  ChainCtxT<false, WPC> grp;
+  grp.red = red;
+  grp.redf = redf;
+  grp.lane = threadIdx.x;
+  const double u = m.value_and_grad(grp, z + (size_t)c * ldz, g + (size_t)c * ldg);
+  if (grp.lane == 0) pe[c] = u;
 }
 
 // Vector slots a persistent launch with one wavefront per workgroup keeps in
 // LDS: the leading slots (the per-leaf ones first) that fit in the CU's
 // 160 KB next to the kernel's static LDS, all of them when the chain's
 // vectors are short.  The host passes nl * Dp * 4 bytes of dynamic LDS.
-inline int persistent_lds_slots(int Dp, int nvec) {
-  const int budget = 160 * 1024 - 2048;
+inline int persistent_lds_slots(int Dp, int nvec, int wpc = 1) {
+  // static LDS of the kernel: a ChainState copy per wave + the group's
+  // reduction slots, and a margin
+  const int budget = 160 * 1024 - 2048 -
+                     wpc * (int)(sizeof(ChainState) + kRedSlots * 8 + NAX_WAVE * 4);
   const int fit = budget / (Dp * 4);
   return fit >= nvec ? nvec : (fit >= V_HOT ? V_HOT : fit);
 }
@@ -797,56 +923,162 @@ inline int persistent_lds_slots(int Dp, int nvec) {
 // vector slots live in LDS for the launch (copied in and back out), so a
 // leaf's vector passes run at LDS rather than L2 latency; for models with
 // long latent vectors.
-template <class M, bool DENSE, int WPB, bool LDSV = false>
-__global__ void __launch_bounds__(64 * WPB)
+// WPC > 1 (with WPB = 1): WPC wavefronts of the workgroup form the chain's
+// group; each wave keeps its own copy of the chain state, wave 0 writes it
+// back.
+template <class M, bool DENSE, int WPB, bool LDSV = false, int WPC = 1>
+__global__ void __launch_bounds__(64 * WPB * WPC)
 persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
                   ChainState* __restrict__ states, float* __restrict__ vec,
                   double* __restrict__ welf, float* __restrict__ zreq, float* __restrict__ gout,
                   double* __restrict__ peout, int64_t* __restrict__ grad_evals, int nl) {
-  __shared__ ChainState sS[WPB];
+  static_assert(WPC == 1 || WPB == 1, "a chain group fills its workgroup");
+  __shared__ ChainState sS[WPB * WPC];
+  __shared__ double nax_red[WPC > 1 ? kRedSlots * WPC : 1];
+  __shared__ float nax_redf[WPC > 1 ? NAX_WAVE * WPC : 1];
   extern __shared__ float nax_slots_lds[];
-  const int c = wave_chain(WPB);
+  const int c = WPC == 1 ? wave_chain(WPB) : (int)blockIdx.x;
   if (c >= cfg.C) return;
-  ChainCtxT<LDSV> cx = make_ctx<DENSE, LDSV>(cfg, c, vec, welf);
+  ChainCtxT<LDSV, WPC> cx = make_ctx<DENSE, LDSV, WPC>(cfg, c, vec, welf);
+  cx.red = nax_red;
+  cx.redf = nax_redf;
   const int nlds = (LDSV && WPB == 1) ? nl : 0;
   if (nlds > 0) {
     const int n = nlds * cfg.Dp;
-    for (int i = cx.lane; i < n; i += NAX_WAVE) nax_slots_lds[i] = cx.vb[i];
+    for (int i = cx.lane; i < n; i += cx.NT) nax_slots_lds[i] = cx.vb[i];
     __syncthreads();
     cx.lb = nax_slots_lds;
     cx.nl = nlds;
   }
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
+  const int wave_lane = threadIdx.x & (NAX_WAVE - 1);
   ChainState& S = sS[threadIdx.x / NAX_WAVE];
-  copy_state(S, states[c], cx.lane);
+  copy_state(S, states[c], wave_lane);
   double pe = peout[c];
   int evals = 0;
   while (advance_chain(cfg, col, target, S, cx, c, zr, gi, pe, evals)) {
-    __threadfence_block();
-    pe = m.value_and_grad(zr, gi, cx.lane);
-    __threadfence_block();
+    chain_sync(cx);
+    pe = m.value_and_grad(cx, zr, gi);
+    chain_sync(cx);
   }
   if (nlds > 0) {
     __syncthreads();
     const int n = nlds * cfg.Dp;
-    for (int i = cx.lane; i < n; i += NAX_WAVE) cx.vb[i] = nax_slots_lds[i];
+    for (int i = cx.lane; i < n; i += cx.NT) cx.vb[i] = nax_slots_lds[i];
   }
-  copy_state(states[c], S, cx.lane);
+  if (cx.lane < NAX_WAVE) copy_state(states[c], S, wave_lane);
   if (cx.lane == 0) {
     peout[c] = pe;
     if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
   }
 }
 
-// ---- wave scans ------------------------------------------------------------------
-__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
-#pragma unroll
-  for (int o = 1; o < NAX_WAVE; o <<= 1) {
-    const double t = __shfl_up(v, o, NAX_WAVE);
-    if (lane >= o) v += t;
+
+}  // namespace nax
+
+// ---- host: persistent launches with a run-time number of waves per chain ---------
+namespace nax {
+
+// Waves per chain the persistent launchers instantiate.
+constexpr int kGroupSizes[] = {1, 2, 4, 8, 16};
+
+// Launches persistent_kernel<M, ., 1, LDSV, wpc> (one chain per workgroup),
+// with the vector slots in LDS when LDSV.  Dense mass runs on one wave.
+template <class M, bool LDSV>
+hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineCfg& cfg,
+                               const Collect& col, int64_t target, ChainState* st, float* vec,
+                               double* wel, float* zreq, float* gout, double* peout,
+                               int64_t* gevals) {
+  if (cfg.dense) wpc = 1;
+#define NAX_GROUP_LAUNCH(DN, W)                                                             \
+  {                                                                                         \
+    const int nl = LDSV ? persistent_lds_slots(cfg.Dp, cfg.nvec, W) : 0;                    \
+    const size_t lds = (size_t)nl * cfg.Dp * sizeof(float);                                 \
+    if (LDSV) {                                                                             \
+      hipError_t e = hipFuncSetAttribute(                                                   \
+          (const void*)persistent_kernel<M, DN, 1, LDSV, W>,                                \
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                            \
+      if (e != hipSuccess) return e;                                                        \
+    }                                                                                       \
+    hipLaunchKernelGGL((persistent_kernel<M, DN, 1, LDSV, W>), dim3(cfg.C), dim3(64 * W),   \
+                       lds, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals, \
+                       nl);                                                                 \
+    return hipGetLastError();                                                               \
   }
-  return v;
+  if (cfg.dense) NAX_GROUP_LAUNCH(true, 1)
+  switch (wpc) {
+    case 1: NAX_GROUP_LAUNCH(false, 1)
+    case 2: NAX_GROUP_LAUNCH(false, 2)
+    case 4: NAX_GROUP_LAUNCH(false, 4)
+    case 8: NAX_GROUP_LAUNCH(false, 8)
+    case 16: NAX_GROUP_LAUNCH(false, 16)
+    default: return hipErrorInvalidValue;
+  }
+#undef NAX_GROUP_LAUNCH
+}
+
+// Autotuner of the waves per chain: from a scratch copy of the current
+// sampler state, each candidate runs the real persistent kernel for `iters`
+// iterations (the same NUTS work, nothing collected) timed with HIP events;
+// the fastest is returned.  The sampler's own state is not touched.  `log`
+// receives "wpc:ms" for every candidate.
+template <class M, bool LDSV>
+int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const ChainState* st,
+                      const float* vec, const double* wel, const float* zreq, const float* gout,
+                      const double* peout, int iters, std::string* log) {
+  if (cfg.dense) return 1;
+  const size_t nst = sizeof(ChainState) * cfg.C;
+  const size_t nvec = sizeof(float) * (size_t)cfg.C * cfg.nvec * cfg.Dp;
+  const size_t nwel = sizeof(double) * (size_t)cfg.C * 3 * cfg.Dp;
+  const size_t nzg = sizeof(float) * (size_t)cfg.C * cfg.Dp;
+  const size_t npe = sizeof(double) * cfg.C;
+  ChainState* st2 = nullptr;
+  float *vec2 = nullptr, *zr2 = nullptr, *g2 = nullptr;
+  double *wel2 = nullptr, *pe2 = nullptr;
+  int64_t* ge2 = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<ChainState> hs(cfg.C);
+  bool ok = hipMalloc(&st2, nst) == hipSuccess && hipMalloc(&vec2, nvec) == hipSuccess &&
+            hipMalloc(&wel2, nwel) == hipSuccess && hipMalloc(&zr2, nzg) == hipSuccess &&
+            hipMalloc(&g2, nzg) == hipSuccess && hipMalloc(&pe2, npe) == hipSuccess &&
+            hipMalloc(&ge2, sizeof(int64_t)) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+            hipMemcpyAsync(hs.data(), st, nst, hipMemcpyDeviceToHost, s) == hipSuccess &&
+            hipStreamSynchronize(s) == hipSuccess;
+  int best = 1;
+  if (ok) {
+    int64_t i0 = 0;
+    for (const ChainState& c : hs) i0 = std::max<int64_t>(i0, c.i);
+    Collect none;
+    memset(&none, 0, sizeof(none));
+    float best_ms = 1e30f;
+    for (int w : kGroupSizes) {
+      float ms = 1e30f;
+      if (hipMemcpyAsync(st2, st, nst, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+          hipMemcpyAsync(vec2, vec, nvec, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+          hipMemcpyAsync(wel2, wel, nwel, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+          hipMemcpyAsync(zr2, zreq, nzg, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+          hipMemcpyAsync(g2, gout, nzg, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+          hipMemcpyAsync(pe2, peout, npe, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+          hipEventRecord(e0, s) == hipSuccess &&
+          launch_chain_groups<M, LDSV>(w, s, m, cfg, none, i0 + iters, st2, vec2, wel2, zr2, g2,
+                                       pe2, ge2) == hipSuccess &&
+          hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess)
+        hipEventElapsedTime(&ms, e0, e1);
+      if (log) *log += " " + std::to_string(w) + ":" + std::to_string(ms);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = w;
+      }
+    }
+  }
+  hipFree(st2); hipFree(vec2); hipFree(wel2); hipFree(zr2); hipFree(g2); hipFree(pe2);
+  hipFree(ge2);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  (void)hipGetLastError();  // a failed candidate leaves no sticky error behind
+  return best;
 }
 
 }  // namespace nax

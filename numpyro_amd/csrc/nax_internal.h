@@ -96,6 +96,11 @@ struct Potential {
   virtual double alg_bytes(int C) const { return 0; }
   virtual double alg_flops(int C) const { return 0; }
   virtual const char* name() const { return "potential"; }
+  // Launch shapes the potential's autotuner chose (empty when it has none).
+  std::string tuning;
+  // Positive latent sites by ExpTransform (kind 0) or SoftplusTransform
+  // (kind 1); false for a model without positive sites.
+  virtual bool set_positive_transform(int kind) { return false; }
   // Latent size of each chain on the device ([C] int32) for ragged models.
   virtual const int* chain_dims() const { return nullptr; }
   // Models whose potential needs no data shared across chains run the whole

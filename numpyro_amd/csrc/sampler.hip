@@ -513,6 +513,33 @@ extern "C" int nax_model_external(nax_sampler* h) {
   return NAX_OK;
 }
 
+extern "C" int nax_tuning(nax_sampler* h, char* buf, int32_t len) {
+  if (!h || !buf || len < 1) { set_error("nax_tuning: null handle or buffer"); return NAX_EINVAL; }
+  const std::string t = h->pot ? h->pot->tuning : std::string();
+  const size_t n = std::min<size_t>(t.size(), (size_t)len - 1);
+  memcpy(buf, t.data(), n);
+  buf[n] = 0;
+  return NAX_OK;
+}
+
+extern "C" int nax_set_positive_transform(nax_sampler* h, int32_t kind) {
+  if (!h) { set_error("null handle"); return NAX_EINVAL; }
+  if (kind != 0 && kind != 1) {
+    set_error("positive transform: 0 (ExpTransform) or 1 (SoftplusTransform)");
+    return NAX_EINVAL;
+  }
+  if (!h->pot) { set_error("no built-in model set"); return NAX_ESTATE; }
+  if (h->initialized) {
+    set_error("nax_set_positive_transform must come before nax_init");
+    return NAX_ESTATE;
+  }
+  if (!h->pot->set_positive_transform(kind)) {
+    set_error(std::string("model '") + h->pot->name() + "' has no positive latent site");
+    return NAX_EINVAL;
+  }
+  return NAX_OK;
+}
+
 // Evaluates the potential for every chain's request.  With timing on, the
 // main kernel of launch k of a batch is bracketed by events ev[2k], ev[2k+1];
 // collect_timing() reads them after the batch has synchronised.

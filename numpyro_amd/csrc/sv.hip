@@ -59,6 +59,7 @@ struct SVModel {
   int noncentred;
   double sigma_rate;  // sigma ~ Exponential(sigma_rate)
   double nu_rate;     // nu ~ Exponential(nu_rate)
+  int softplus;       // positive sites by SoftplusTransform instead of ExpTransform
 
   // T-dependent constant part of U and of dU/dnu for the StudentT block.
   __device__ void studentt_consts(double nu, double& u_const, double& dnu_const) const {
@@ -66,13 +67,17 @@ struct SVModel {
     dnu_const = T * (0.5 / nu + 0.5 * digamma_pos(0.5 * nu) - 0.5 * digamma_pos(0.5 * (nu + 1.0)));
   }
 
-  __device__ double value_and_grad(const float* z, float* g, int lane) const {
-    return noncentred ? nc(z, g, lane) : centred(z, g, lane);
+  template <class G>
+  __device__ double value_and_grad(const G& grp, const float* z, float* g) const {
+    return noncentred ? nc(grp, z, g) : centred(grp, z, g);
   }
 
-  __device__ double centred(const float* z, float* g, int lane) const {
+  template <class G>
+  __device__ double centred(const G& grp, const float* z, float* g) const {
+    const int lane = grp.lane;
     const double u_nu = z[0], u_sig = z[T + 1];
-    const double nu = exp(u_nu), sig = exp(u_sig);
+    const PosTf tn = positive_tf(u_nu, softplus), ts = positive_tf(u_sig, softplus);
+    const double nu = tn.x, sig = ts.x;
     const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
     const float inv_s2 = (float)(1.0 / (sig * sig));
     const float inv_nu = 1.0f / nuf;
@@ -80,7 +85,7 @@ struct SVModel {
     // lane, scale once (nu is constant within an evaluation)
     double U = 0.0, S2 = 0.0, asum = 0.0, bsum = 0.0;
 #pragma unroll 4
-    for (int t = lane; t < T; t += NAX_WAVE) {
+    for (int t = lane; t < T; t += G::NT) {
       const float s = z[1 + t];
       const float dprev = s - (t > 0 ? z[t] : 0.0f);
       const float dnext = (t + 1 < T) ? (z[2 + t] - s) : 0.0f;
@@ -94,35 +99,44 @@ struct SVModel {
       bsum += (double)b;
       g[1 + t] = (dprev - dnext) * inv_s2 + 1.0f - nu1 * b;
     }
-    U = wave_sum(U);
-    S2 = wave_sum(S2);
-    const double dnu = 0.5 * wave_sum(asum) - 0.5 * (double)nu1 * wave_sum(bsum) / nu;
+    double v[4] = {U, S2, asum, bsum};
+    chain_sum_n(grp, v);
+    U = v[0];
+    S2 = v[1];
+    const double dnu = 0.5 * v[2] - 0.5 * (double)nu1 * v[3] / nu;
     double uc, dnc;
     studentt_consts(nu, uc, dnc);
-    U += uc + T * (u_sig + 0.5 * kLog2Pi)             // GRW normaliser
-         - log(sigma_rate) + sigma_rate * sig - u_sig    // Exponential(a) + log|J|
-         - log(nu_rate) + nu_rate * nu - u_nu;           // Exponential(b) + log|J|
+    U += uc + T * ((softplus ? log(sig) : u_sig) + 0.5 * kLog2Pi)  // GRW normaliser
+         - log(sigma_rate) + sigma_rate * sig - ts.ladj     // Exponential(a) + log|J|
+         - log(nu_rate) + nu_rate * nu - tn.ladj;           // Exponential(b) + log|J|
     if (lane == 0) {
-      g[0] = (float)(nu * (dnu + dnc + nu_rate) - 1.0);
-      g[T + 1] = (float)(-S2 / (sig * sig) + T + sigma_rate * sig - 1.0);
+      // dU/du = dU/dx dx/du - d log|J| / du for u = log nu, log sigma (or
+      // softplus^-1); for ExpTransform sig (-S2 / sig^3 + T / sig + a) - 1
+      g[0] = (float)(tn.dx * (dnu + dnc + nu_rate) - tn.dladj);
+      g[T + 1] = softplus
+          ? (float)(ts.dx * (-S2 / (sig * sig * sig) + T / sig + sigma_rate) - ts.dladj)
+          : (float)(-S2 / (sig * sig) + T + sigma_rate * sig - 1.0);
     }
     return U;
   }
 
-  __device__ double nc(const float* z, float* g, int lane) const {
+  template <class G>
+  __device__ double nc(const G& grp, const float* z, float* g) const {
+    const int lane = grp.lane;
     const double u_nu = z[0], u_sig = z[1];
-    const double nu = exp(u_nu), sig = exp(u_sig);
+    const PosTf tn = positive_tf(u_nu, softplus), ts = positive_tf(u_sig, softplus);
+    const double nu = tn.x, sig = ts.x;
     const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
     const float* zeta = z + 2;
     float* gz = g + 2;
-    const int L = (T + NAX_WAVE - 1) / NAX_WAVE;
+    const int L = (T + G::NT - 1) / G::NT;
     const int t0 = lane * L;
     const int t1 = min(T, t0 + L);
     // pass 1: segment sums -> prefix checkpoints
     double seg = 0.0;
 #pragma unroll 8
     for (int t = t0; t < t1; ++t) seg += (double)zeta[t];
-    const double cp = wave_inclusive_scan(seg, lane) - seg;
+    const double cp = chain_inclusive_scan(grp, seg) - seg;
     // pass 2: StudentT terms and dU/ds_t = 1 - (nu+1) y^2 / (nu + y^2)
     const float inv_nu = 1.0f / nuf;
     double U = 0.0, asum = 0.0, bsum = 0.0, A = 0.0, Asc = 0.0, Z2 = 0.0;
@@ -144,8 +158,8 @@ struct SVModel {
       Asc += (double)ds * c;
       Z2 += (double)zt * (double)zt;
     }
-    const double Atot = wave_sum(A);
-    const double after = Atot - wave_inclusive_scan(A, lane);  // sum over later segments
+    const double Atot = chain_sum(grp, A);
+    const double after = Atot - chain_inclusive_scan(grp, A);  // sum over later segments
     // pass 3: dU/dzeta_t = zeta_t + sigma * (suffix of dU/ds from t)
     double P = 0.0;
     c = cp;
@@ -160,17 +174,20 @@ struct SVModel {
       gz[t] = (float)((double)zt + sig * (after + A - P));
       P += (double)ds;
     }
-    U = wave_sum(U) + 0.5 * wave_sum(Z2);
-    const double dnu = 0.5 * wave_sum(asum) - 0.5 * (double)nu1 * wave_sum(bsum) / nu;
-    const double asc = wave_sum(Asc);
+    double v[5] = {U, Z2, asum, bsum, Asc};
+    chain_sum_n(grp, v);
+    U = v[0] + 0.5 * v[1];
+    const double dnu = 0.5 * v[2] - 0.5 * (double)nu1 * v[3] / nu;
+    const double asc = v[4];
     double uc, dnc;
     studentt_consts(nu, uc, dnc);
     U += uc + T * 0.5 * kLog2Pi                         // N(0,1) normaliser of zeta
-         - log(sigma_rate) + sigma_rate * sig - u_sig
-         - log(nu_rate) + nu_rate * nu - u_nu;
+         - log(sigma_rate) + sigma_rate * sig - ts.ladj
+         - log(nu_rate) + nu_rate * nu - tn.ladj;
     if (lane == 0) {
-      g[0] = (float)(nu * (dnu + dnc + nu_rate) - 1.0);
-      g[1] = (float)(sig * asc + sigma_rate * sig - 1.0);
+      g[0] = (float)(tn.dx * (dnu + dnc + nu_rate) - tn.dladj);
+      g[1] = softplus ? (float)(ts.dx * (asc + sigma_rate) - ts.dladj)
+                      : (float)(sig * asc + sigma_rate * sig - 1.0);
     }
     return U;
   }
@@ -182,6 +199,10 @@ struct SVPotential : Potential {
   ~SVPotential() override { hipFree(r); }
   int dim() const override { return m.T + 2; }
   const char* name() const override { return "sv"; }
+  bool set_positive_transform(int kind) override {
+    m.softplus = kind == 1;
+    return true;
+  }
   bool persistent() const override { return true; }
   double alg_bytes(int C) const override { return (double)C * 4.0 * (2.0 * (m.T + 2) + m.T); }
 
@@ -196,28 +217,24 @@ struct SVPotential : Potential {
   hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
                             int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
                             float* gout, double* peout, int64_t* gevals) override {
-    // one chain per workgroup: each chain gets a CU (and its LDS) to itself
-    // while C <= #CUs
-    const int nl = persistent_lds_slots(cfg.Dp, cfg.nvec);
-    const size_t lds = (size_t)nl * cfg.Dp * sizeof(float);
-    hipError_t e = cfg.dense
-        ? hipFuncSetAttribute((const void*)persistent_kernel<SVModel, true, 1, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
-        : hipFuncSetAttribute((const void*)persistent_kernel<SVModel, false, 1, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+    // one chain per workgroup, its vector slots in LDS; the waves per chain
+    // (a chain's time steps and vector passes split over 64 wpc threads)
+    // are picked on the first launch by timing a few iterations of each
+    // candidate from a scratch copy of the state
+    if (wpc == 0 || tuned_chains != cfg.C) {
+      std::string log;
+      wpc = tune_chain_groups<SVModel, true>(s, m, cfg, st, vec, wel, zreq, gout, peout, 2, &log);
+      tuned_chains = cfg.C;
+      tuning = "sv: " + std::to_string(wpc) + " waves per chain (ms for 2 iterations:" + log +
+               ")";
+    }
     mark_begin(s);
-    if (cfg.dense)
-      hipLaunchKernelGGL((persistent_kernel<SVModel, true, 1, true>), dim3(cfg.C), dim3(64),
-                         lds, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals,
-                         nl);
-    else
-      hipLaunchKernelGGL((persistent_kernel<SVModel, false, 1, true>), dim3(cfg.C), dim3(64),
-                         lds, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals,
-                         nl);
+    hipError_t e = launch_chain_groups<SVModel, true>(wpc, s, m, cfg, col, target, st, vec, wel,
+                                                     zreq, gout, peout, gevals);
     mark_end(s);
-    return hipGetLastError();
+    return e;
   }
+  int wpc = 0, tuned_chains = 0;
 };
 
 Potential* make_sv(const float* returns, int T, int noncentred, double sigma_rate, double nu_rate,
@@ -246,6 +263,7 @@ Potential* make_sv(const float* returns, int T, int noncentred, double sigma_rat
   p->m.noncentred = noncentred;
   p->m.sigma_rate = sigma_rate;
   p->m.nu_rate = nu_rate;
+  p->m.softplus = 0;
   return p;
 }
 

@@ -230,6 +230,12 @@ class Engine:
         _lib.check(self.lib.nax_set_progress(self.h, self._progress, None, int(every_ticks)),
                    "progress")
 
+    def tuning(self):
+        """The launch shapes the model's autotuner chose, as text."""
+        buf = C.create_string_buffer(1024)
+        _lib.check(self.lib.nax_tuning(self.h, buf, len(buf)), "nax_tuning")
+        return buf.value.decode()
+
     def timing(self, enable):
         ms, n = C.c_double(), C.c_int64()
         _lib.check(self.lib.nax_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)),

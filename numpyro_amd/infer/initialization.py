@@ -10,6 +10,8 @@ unconstrained coordinate at 0 (``initialization.py:132-137``).
 """
 import torch
 
+from ..models import unconstrain
+
 
 class init_to_value:
     """``init_to_value(site=None, values={})``."""
@@ -31,9 +33,8 @@ class init_to_value:
             raise ValueError(f"init_to_value: no latent site named {sorted(unknown)}")
         for name, size, tf in site_sizes:
             if name in self.values:
-                v = torch.as_tensor(self.values[name], dtype=torch.float64).cpu()
-                if tf == "exp":
-                    v = torch.log(v)
+                v = unconstrain(torch.as_tensor(self.values[name], dtype=torch.float64).cpu(),
+                                tf)
                 z0[:, o:o + size] = v.reshape(-1, size).to(torch.float32)
             o += size
         return z0

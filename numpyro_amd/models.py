@@ -25,6 +25,37 @@ def _check_binary_obs(y):
     return ok
 
 
+POSITIVE_TRANSFORMS = {"exp": 0, "softplus": 1}
+
+
+def constrain(v, tf):
+    """Constrained value of an unconstrained site: ExpTransform ("exp",
+    transforms.py:606-652), SoftplusTransform ("softplus",
+    transforms.py:1156-1175) or the identity (None)."""
+    if tf == "exp":
+        return torch.exp(v)
+    if tf == "softplus":
+        return torch.nn.functional.softplus(v)
+    return v
+
+
+def unconstrain(v, tf):
+    """Inverse of ``constrain``; softplus^-1(x) = log(-expm1(-x)) + x
+    (``_softplus_inv``, transforms.py:1152-1153)."""
+    if tf == "exp":
+        return torch.log(v)
+    if tf == "softplus":
+        return torch.log(-torch.expm1(-v)) + v
+    return v
+
+
+def _positive_kind(positive_transform):
+    if positive_transform not in POSITIVE_TRANSFORMS:
+        raise ValueError("positive_transform must be 'exp' (biject_to(positive)) or "
+                         "'softplus' (biject_to(softplus_positive))")
+    return positive_transform
+
+
 class Model:
     """Base class: a potential U(z) over a flat latent of size ``dim``."""
 
@@ -41,7 +72,7 @@ class Model:
             v = z[..., o:o + size]
             if size == 1:
                 v = v[..., 0]
-            out[name] = torch.exp(v) if tf == "exp" else v
+            out[name] = constrain(v, tf)
             o += size
         return out
 
@@ -176,9 +207,12 @@ class StochasticVolatility(Model):
     ``sigma_rate`` / ``nu_rate``),
     r ~ StudentT(nu, 0, exp(s)).  Latent (sorted site names, positive sites
     through ExpTransform): ``[log nu, s (T), log sigma]``; with
-    ``noncentred=True`` ``[log nu, log sigma, zeta (T)]``, s = sigma cumsum(zeta)."""
+    ``noncentred=True`` ``[log nu, log sigma, zeta (T)]``, s = sigma cumsum(zeta).
+    ``positive_transform="softplus"`` unconstrains sigma and nu by
+    SoftplusTransform instead (biject_to(softplus_positive))."""
 
-    def __init__(self, returns, noncentred=False, sigma_rate=50.0, nu_rate=0.1):
+    def __init__(self, returns, noncentred=False, sigma_rate=50.0, nu_rate=0.1,
+                 positive_transform="exp"):
         r = torch.as_tensor(returns)
         if r.ndim != 1:
             raise ValueError("returns must be a vector")
@@ -190,15 +224,18 @@ class StochasticVolatility(Model):
         if not (self.sigma_rate > 0 and self.nu_rate > 0 and math.isfinite(self.sigma_rate)
                 and math.isfinite(self.nu_rate)):
             raise ValueError("sigma_rate and nu_rate must be finite and positive")
+        self.positive_transform = tf = _positive_kind(positive_transform)
         if self.noncentred:
-            self.site_sizes = (("nu", 1, "exp"), ("sigma", 1, "exp"), ("zeta", self.T, None))
+            self.site_sizes = (("nu", 1, tf), ("sigma", 1, tf), ("zeta", self.T, None))
         else:
-            self.site_sizes = (("nu", 1, "exp"), ("s", self.T, None), ("sigma", 1, "exp"))
+            self.site_sizes = (("nu", 1, tf), ("s", self.T, None), ("sigma", 1, tf))
 
     def attach(self, handle):
-        _lib.check(_lib.load().nax_model_sv_ex(handle, _lib.ptr(self.r), self.T,
-                                               int(self.noncentred), self.sigma_rate,
-                                               self.nu_rate), "sv model")
+        lib = _lib.load()
+        _lib.check(lib.nax_model_sv_ex(handle, _lib.ptr(self.r), self.T, int(self.noncentred),
+                                       self.sigma_rate, self.nu_rate), "sv model")
+        _lib.check(lib.nax_set_positive_transform(
+            handle, POSITIVE_TRANSFORMS[self.positive_transform]), "positive transform")
 
     def postprocess(self, z):
         out = super().postprocess(z)
@@ -217,9 +254,11 @@ class HierarchicalLogistic(Model):
     eta ~ Normal(0,1)^D_i, mu ~ Normal(0,1), tau ~ HalfCauchy(1),
     beta = mu + tau eta, y ~ Bernoulli(logits = X_i beta).  A chain's latent
     (sorted site names) is ``[eta (D_i), mu, log tau]``, zero-padded to
-    ``dim = max D_i + 2``; chain c belongs to group c // chains_per_group."""
+    ``dim = max D_i + 2``; chain c belongs to group c // chains_per_group.
+    ``positive_transform="softplus"`` unconstrains tau by SoftplusTransform
+    (the latent then holds softplus^-1(tau) instead of log tau)."""
 
-    def __init__(self, Xs, ys, chains_per_group=4):
+    def __init__(self, Xs, ys, chains_per_group=4, positive_transform="exp"):
         if len(Xs) != len(ys) or not Xs:
             raise ValueError("Xs and ys must be equally long, non-empty lists")
         xs = [torch.as_tensor(x, dtype=torch.float32) for x in Xs]
@@ -244,11 +283,15 @@ class HierarchicalLogistic(Model):
         self.row_off = torch.tensor(row, dtype=torch.int64, device="cuda")
         self.dims_t = torch.tensor(self.dims, dtype=torch.int32, device="cuda")
         self.site_sizes = ()
+        self.positive_transform = _positive_kind(positive_transform)
 
     def attach(self, handle):
-        _lib.check(_lib.load().nax_model_hier(
+        lib = _lib.load()
+        _lib.check(lib.nax_model_hier(
             handle, _lib.ptr(self.X), _lib.ptr(self.y), _lib.ptr(self.row_off),
             _lib.ptr(self.elt_off), _lib.ptr(self.dims_t), self.G, self.K), "hierarchical model")
+        _lib.check(lib.nax_set_positive_transform(
+            handle, POSITIVE_TRANSFORMS[self.positive_transform]), "positive transform")
 
     def postprocess(self, z):
         """{"eta": [..., max D] (zero past D_i), "mu", "tau"} per chain."""
@@ -259,7 +302,7 @@ class HierarchicalLogistic(Model):
         ut = torch.gather(z, -1, (idx + 1).expand(z.shape[:-1] + (1,))).squeeze(-1)
         ar = torch.arange(z.shape[-1], device=z.device)
         eta = torch.where(ar < idx, z, torch.zeros_like(z))[..., :max(self.dims)]
-        return {"eta": eta, "mu": mu, "tau": torch.exp(ut)}
+        return {"eta": eta, "mu": mu, "tau": constrain(ut, self.positive_transform)}
 
     def algorithmic_bytes(self, num_chains):
         return 4.0 * sum(n * d + n for n, d in zip(self.rows, self.dims))

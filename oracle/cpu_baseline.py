@@ -34,3 +34,23 @@ def nuts_logistic(X, y, z0, keys, num_warmup, num_samples, max_depth=10, step_si
                                   ns.ctypes.data, C.byref(ge), threads)
     assert rc == 0
     return z_out, ns, ge.value
+
+
+def logistic_batch_grad(X, y, z, prior_scale=1.0, threads=0):
+    """U [C] and dU/dz [C, D] (float64) of the logistic model for the chains
+    at z [C, D], one multi-threaded pass over X for all chains."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    Cn, D = z.shape
+    U = np.zeros(Cn)
+    g = np.zeros((Cn, D))
+    lib = C.CDLL(LIB)
+    f = lib.cpu_logistic_batch_grad
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.c_int]
+    rc = f(X.ctypes.data, y.ctypes.data, X.shape[0], D, prior_scale, Cn, z.ctypes.data,
+           U.ctypes.data, g.ctypes.data, threads)
+    assert rc == 0
+    return U, g

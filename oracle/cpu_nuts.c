@@ -308,3 +308,102 @@ int cpu_nuts_logistic(const float* X, const float* y, int64_t N, int D, double p
   if (grad_evals) *grad_evals = total_evals;
   return 0;
 }
+
+/* The vectorized chain method's per-leapfrog work on the CPU (mcmc.py:720-724
+ * vmaps the chains, so each leapfrog step of the batch is one X Z^T and one
+ * X^T R over all C chains): U and dU/dz of the logistic model for C chains at
+ * z [C][D], with every thread taking a contiguous block of rows and passing
+ * over each of its rows once for all chains (logits for all chains, then the
+ * rank-1 update of its private gradient block).  Thread partials are summed
+ * in thread order.  U [C] fp64, g [C][D] fp64.  Returns 0. */
+/* fast-math for this function only: lets gcc call the vector expf / log1pf
+ * of glibc's libmvec for the per-(row, chain) terms */
+__attribute__((optimize("fast-math")))
+int cpu_logistic_batch_grad(const float* X, const float* y, int64_t N, int D, double prior_scale,
+                            int Cn, const float* z, double* U, double* g, int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+  const int T = threads > 0 ? threads : omp_get_max_threads();
+#else
+  const int T = 1;
+#endif
+  const int Cp = (Cn + 15) / 16 * 16;                              /* chains padded to 16 */
+  float* zt = (float*)calloc((size_t)D * Cp, sizeof(float));       /* [D][Cp] */
+  float* gp = (float*)calloc((size_t)T * D * Cp, sizeof(float));  /* per-thread [D][Cp] */
+  double* up = (double*)calloc((size_t)T * Cn, sizeof(double));
+  for (int c = 0; c < Cn; ++c)
+    for (int d = 0; d < D; ++d) zt[(size_t)d * Cp + c] = z[(size_t)c * D + d];
+#pragma omp parallel num_threads(T)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+    const int64_t r0 = N * t / T, r1 = N * (t + 1) / T;
+    float* gt = gp + (size_t)t * D * Cp;
+    double* ut = up + (size_t)t * Cn;
+    /* four rows at a time, chains in blocks of 16 held in registers across
+     * the d loop: each zt / gradient row is loaded once per 4 rows */
+    float* restrict l = (float*)malloc(sizeof(float) * 4 * Cp);
+    for (int64_t n0 = r0; n0 < r1; n0 += 4) {
+      const int nr = (int)(r1 - n0 < 4 ? r1 - n0 : 4);
+      const float* xq[4];
+      for (int q = 0; q < 4; ++q) xq[q] = X + (n0 + (q < nr ? q : 0)) * D;
+      for (int c0 = 0; c0 < Cp; c0 += 16) {
+        float acc[4][16];
+        for (int q = 0; q < 4; ++q)
+          for (int j = 0; j < 16; ++j) acc[q][j] = 0.0f;
+        for (int d = 0; d < D; ++d) {
+          const float* restrict zd = zt + (size_t)d * Cp + c0;
+          for (int q = 0; q < 4; ++q) {
+            const float xv = xq[q][d];
+            for (int j = 0; j < 16; ++j) acc[q][j] += xv * zd[j];
+          }
+        }
+        for (int q = 0; q < 4; ++q)
+          for (int j = 0; j < 16; ++j) l[(size_t)q * Cp + c0 + j] = acc[q][j];
+      }
+      for (int q = 0; q < nr; ++q) {
+        const float yv = y[n0 + q];
+        float* restrict lq = l + (size_t)q * Cp;
+        for (int c = 0; c < Cn; ++c) {
+          const float lc = lq[c];
+          ut[c] += (double)(fmaxf(lc, 0.0f) + log1pf(expf(-fabsf(lc))) - yv * lc);
+          lq[c] = 1.0f / (1.0f + expf(-lc)) - yv;
+        }
+        for (int c = Cn; c < Cp; ++c) lq[c] = 0.0f;
+      }
+      for (int q = nr; q < 4; ++q)
+        for (int c = 0; c < Cp; ++c) l[(size_t)q * Cp + c] = 0.0f;
+      for (int d = 0; d < D; ++d) {
+        float* restrict gd = gt + (size_t)d * Cp;
+        const float x0 = xq[0][d], x1 = xq[1][d], x2 = xq[2][d], x3 = xq[3][d];
+        const float* restrict l0 = l;
+        const float* restrict l1 = l + Cp;
+        const float* restrict l2 = l + 2 * (size_t)Cp;
+        const float* restrict l3 = l + 3 * (size_t)Cp;
+        for (int c = 0; c < Cp; ++c) gd[c] += x0 * l0[c] + x1 * l1[c] + x2 * l2[c] + x3 * l3[c];
+      }
+    }
+    free(l);
+  }
+  const double inv_s2 = 1.0 / (prior_scale * prior_scale);
+  const double lnorm = log(sqrt(2.0 * M_PI) * prior_scale);
+  for (int c = 0; c < Cn; ++c) {
+    double u = 0.0;
+    for (int t = 0; t < T; ++t) u += up[(size_t)t * Cn + c];
+    for (int d = 0; d < D; ++d) {
+      const double zv = z[(size_t)c * D + d];
+      double s = 0.0;
+      for (int t = 0; t < T; ++t) s += gp[((size_t)t * D + d) * Cp + c];
+      g[(size_t)c * D + d] = s + zv * inv_s2;
+      u += 0.5 * zv * zv * inv_s2 + lnorm;
+    }
+    U[c] = u;
+  }
+  free(zt);
+  free(gp);
+  free(up);
+  return 0;
+}

@@ -44,14 +44,24 @@ SV_SIGMA_RATE = 50.0
 SV_NU_RATE = 0.1
 
 
-def sv_potential(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
+def _positive(u, positive):
+    """(x, log|dx/du|) of a positive site: ExpTransform (transforms.py:606-652)
+    or SoftplusTransform (transforms.py:1156-1175)."""
+    if positive == "exp":
+        return np.exp(u), u
+    if positive == "softplus":
+        return dist.softplus_transform(u)
+    raise ValueError(positive)
+
+
+def sv_potential(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE, positive="exp"):
     z = np.asarray(z, dtype=np.float64)
     T = returns.shape[0]
     u_nu, s, u_sig = z[0], z[1:1 + T], z[1 + T]
-    nu, sig = np.exp(u_nu), np.exp(u_sig)
-    lp = dist.exponential_log_prob(sig, sigma_rate) + u_sig
+    (nu, j_nu), (sig, j_sig) = _positive(u_nu, positive), _positive(u_sig, positive)
+    lp = dist.exponential_log_prob(sig, sigma_rate) + j_sig
     lp += dist.gaussian_random_walk_log_prob(s, sig)
-    lp += dist.exponential_log_prob(nu, nu_rate) + u_nu
+    lp += dist.exponential_log_prob(nu, nu_rate) + j_nu
     lp += np.sum(dist.student_t_log_prob(returns, nu, 0.0, np.exp(s)))
     return -lp
 
@@ -83,14 +93,14 @@ def sv_potential_grad(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
 # ---- non-centred hierarchical logistic regression, one group ----------------
 # z = [eta_0..eta_{D-1}, mu, log tau]  (sorted: "eta", "mu", "tau")
 
-def hier_potential(z, X, y):
+def hier_potential(z, X, y, positive="exp"):
     z = np.asarray(z, dtype=np.float64)
     D = X.shape[1]
     eta, mu, u_tau = z[:D], z[D], z[D + 1]
-    tau = np.exp(u_tau)
+    tau, j_tau = _positive(u_tau, positive)
     beta = mu + tau * eta
     lp = np.sum(dist.normal_log_prob(eta)) + dist.normal_log_prob(mu)
-    lp += dist.half_cauchy_log_prob(tau, 1.0) + u_tau
+    lp += dist.half_cauchy_log_prob(tau, 1.0) + j_tau
     lp += np.sum(dist.bernoulli_logits_log_prob(np.asarray(X, np.float64) @ beta, y))
     return -lp
 
@@ -128,15 +138,15 @@ def central_fd(f, z, eps=1e-5):
 # of examples/stochastic_volatility.py:58-66 (s_0 ~ N(0, sigma), increments
 # N(0, sigma)).
 
-def sv_nc_potential(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE):
+def sv_nc_potential(z, returns, sigma_rate=SV_SIGMA_RATE, nu_rate=SV_NU_RATE, positive="exp"):
     z = np.asarray(z, dtype=np.float64)
     T = returns.shape[0]
     u_nu, u_sig, zeta = z[0], z[1], z[2:2 + T]
-    nu, sig = np.exp(u_nu), np.exp(u_sig)
+    (nu, j_nu), (sig, j_sig) = _positive(u_nu, positive), _positive(u_sig, positive)
     s = sig * np.cumsum(zeta)
-    lp = dist.exponential_log_prob(sig, sigma_rate) + u_sig
+    lp = dist.exponential_log_prob(sig, sigma_rate) + j_sig
     lp += np.sum(dist.normal_log_prob(zeta))
-    lp += dist.exponential_log_prob(nu, nu_rate) + u_nu
+    lp += dist.exponential_log_prob(nu, nu_rate) + j_nu
     lp += np.sum(dist.student_t_log_prob(returns, nu, 0.0, np.exp(s)))
     return -lp
 

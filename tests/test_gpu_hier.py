@@ -123,3 +123,38 @@ def test_hier_group_posterior_vs_torch_potential():
     mu_t = m2.get_samples(group_by_chain=True)[..., D].cpu().numpy().astype(np.float64)
     se = np.sqrt(mu_h.var() / effective_sample_size(mu_h) + mu_t.var() / effective_sample_size(mu_t))
     assert abs(mu_h.mean() - mu_t.mean()) < 4 * se, (mu_h.mean(), mu_t.mean(), se)
+
+
+def test_hier_softplus_potential_grad():
+    # tau unconstrained by SoftplusTransform (transforms.py:1156-1175): U
+    # against the float64 oracle, dU against central differences of it
+    from numpyro_amd import _lib
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import HierarchicalLogistic
+    shapes = [(63, 4), (200, 9), (1000, 33), (65, 64)]
+    K = 2
+    Xs, ys = _groups(shapes, 17)
+    m = HierarchicalLogistic(Xs, ys, chains_per_group=K, positive_transform="softplus")
+    C = len(shapes) * K
+    eng = Engine(m, C, "NUTS", 0)
+    rng = np.random.default_rng(6)
+    Z = np.zeros((C, m.dim), np.float32)
+    for c in range(C):
+        d = shapes[c // K][1]
+        Z[c, :d + 2] = 0.5 * rng.normal(size=d + 2)
+        Z[c, d + 1] = rng.uniform(-3, 3)
+    zp = torch.zeros((C, eng_stride(eng)), device="cuda")
+    zp[:, :m.dim] = torch.from_numpy(Z).cuda()
+    pe = torch.empty(C, dtype=torch.float64, device="cuda")
+    g = torch.zeros_like(zp)
+    _lib.check(_lib.load().nax_potential_grad(eng.h, _lib.ptr(zp), zp.shape[1], C, _lib.ptr(pe),
+                                              _lib.ptr(g)))
+    for c in range(C):
+        i, d = c // K, shapes[c // K][1]
+        zc = Z[c, :d + 2].astype(np.float64)
+        u = om.hier_potential(zc, Xs[i], ys[i], positive="softplus")
+        gr = om.central_fd(lambda v: om.hier_potential(v, Xs[i], ys[i], positive="softplus"),
+                           zc, eps=1e-6)
+        np.testing.assert_allclose(pe[c].item(), u, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(g[c, :d + 2].cpu().numpy(), gr, rtol=1e-4,
+                                   atol=1e-5 * (np.abs(gr).max() + 1))

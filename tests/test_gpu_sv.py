@@ -103,3 +103,35 @@ def test_sv_centred_vs_noncentred_posterior():
     for (m0, s0, e0), (m1, s1, e1) in zip(res[False], res[True]):
         mcse = np.sqrt(s0 ** 2 / e0 + s1 ** 2 / e1)
         assert abs(m0 - m1) < 4 * mcse, (m0, m1, mcse)
+
+
+@pytest.mark.parametrize("noncentred", [False, True])
+@pytest.mark.parametrize("T, C", [(300, 6), (65, 3)])
+def test_sv_softplus_potential_grad(noncentred, T, C):
+    # sigma and nu unconstrained by SoftplusTransform (transforms.py:1156-1175):
+    # U against the float64 oracle, dU against central differences of it
+    from numpyro_amd import _lib
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import StochasticVolatility
+    r = _returns(T, 70 + T)
+    Z = _points(T, C, noncentred, 3 * T + C)
+    pos = [0, 1] if noncentred else [0, T + 1]
+    # unconstrained coordinates: softplus^-1 of the same sigma, nu values
+    x = np.exp(Z[:, pos].astype(np.float64))
+    Z[:, pos] = (np.log(-np.expm1(-x)) + x).astype(np.float32)
+    eng = Engine(StochasticVolatility(r, noncentred=noncentred, positive_transform="softplus"),
+                 C, "NUTS", 0)
+    z = torch.from_numpy(Z).cuda()
+    pe = torch.empty(C, dtype=torch.float64, device="cuda")
+    g = torch.empty((C, T + 2), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.load().nax_potential_grad(eng.h, _lib.ptr(z), T + 2, C, _lib.ptr(pe),
+                                              _lib.ptr(g)))
+    f = om.sv_nc_potential if noncentred else om.sv_potential
+    r64 = r.astype(np.float64)
+    for c in range(C):
+        zc = Z[c].astype(np.float64)
+        u = f(zc, r64, positive="softplus")
+        gr = om.central_fd(lambda v: f(v, r64, positive="softplus"), zc, eps=1e-6)
+        np.testing.assert_allclose(pe[c].item(), u, rtol=2e-6, atol=1e-3)
+        np.testing.assert_allclose(g[c].cpu().numpy(), gr, rtol=2e-4,
+                                   atol=2e-5 * (np.abs(gr).max() + 1))

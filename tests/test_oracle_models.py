@@ -108,3 +108,27 @@ def test_digamma_reference_values():
     # psi(1) = -gamma, psi(1/2) = -gamma - 2 ln 2 (closed forms)
     assert_allclose(digamma(1.0), -0.5772156649015329, rtol=1e-14)
     assert_allclose(digamma(0.5), -0.5772156649015329 - 2 * np.log(2), rtol=1e-14)
+
+
+def test_softplus_potentials_are_the_exp_potentials_under_the_change_of_variables():
+    # With x = softplus(u) = exp(v), v = log softplus(u): the two unconstrained
+    # potentials differ exactly by the two log-Jacobians,
+    # U_softplus(u) = U_exp(v) + v + softplus(-u)   (transforms.py:606-652, 1156-1175)
+    rng = np.random.default_rng(3)
+    T = 40
+    r = rng.normal(size=T) * 0.02
+    for fn, pos in ((m.sv_potential, [0, T + 1]), (m.sv_nc_potential, [0, 1])):
+        for _ in range(3):
+            z = rng.normal(size=T + 2) * 0.5
+            z[pos] = rng.normal(size=2)
+            v = z.copy()
+            v[pos] = np.log(d.softplus(z[pos]))
+            want = fn(v, r) + np.sum(v[pos]) + np.sum(d.softplus(-z[pos]))
+            assert_allclose(fn(z, r, positive="softplus"), want, rtol=1e-12)
+    X = rng.normal(size=(30, 4))
+    y = (rng.uniform(size=30) < 0.5).astype(np.float64)
+    z = rng.normal(size=6)
+    v = z.copy()
+    v[5] = np.log(d.softplus(z[5]))
+    assert_allclose(m.hier_potential(z, X, y, positive="softplus"),
+                    m.hier_potential(v, X, y) + v[5] + d.softplus(-z[5]), rtol=1e-12)
