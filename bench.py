@@ -43,10 +43,11 @@ MIN_ADAPT = 150
 
 
 def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True, imm=None,
-             heuristic=False):
+             heuristic=False, dense=False):
     import torch
     from numpyro_amd.infer.hmc import Engine
-    eng = Engine(model, C, "NUTS", num_warmup, find_heuristic_step_size=heuristic)
+    eng = Engine(model, C, "NUTS", num_warmup, find_heuristic_step_size=heuristic,
+                 dense_mass=dense)
     eng.init(z0, keys, imm)
     t_start = time.perf_counter()
     eng.set_progress(lambda ticks, pending: log(
@@ -161,6 +162,9 @@ def main():
                     help="timed iterations of the plain FP32 form (0: skip)")
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of CPU-baseline work (0: skip)")
+    ap.add_argument("--dense-mass", action="store_true",
+                    help="logistic workloads: dense_mass=True, initial inverse_mass_matrix "
+                         "the Laplace covariance")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); chains "
@@ -186,18 +190,19 @@ def main():
     log(f"workload {args.config}: N={N} D={D} chains={C} x_dtype={args.x_dtype}")
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
     t0 = time.perf_counter()
-    z0, imm = workloads.laplace_init(X, y, C)
+    z0, imm = workloads.laplace_init(X, y, C, full_cov=args.dense_mass)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+    imd = torch.diagonal(imm) if imm.ndim == 2 else imm
     log(f"laplace init: {setup_s:.1f}s, inverse mass diagonal in "
-        f"[{float(imm.min()):.3g}, {float(imm.max()):.3g}]")
+        f"[{float(imd.min()):.3g}, {float(imd.max()):.3g}]")
 
     model = LogisticRegression(
         X, y, x_dtype=args.x_dtype,
         kernel=args.bf16_kernel if args.x_dtype == "bf16" else "auto")
     nw = max(args.warmup, MIN_ADAPT)
     eng, out, wall, warm_s, kms, kn = run_form(model, C, nw, args.steps, z0, keys,
-                                               imm=imm, heuristic=True)
+                                               imm=imm, heuristic=True, dense=args.dense_mass)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
     div = int(out["diverging"].sum())
@@ -222,7 +227,7 @@ def main():
     if args.fp32_steps > 0 and args.x_dtype != "fp32" and valid:
         m32 = LogisticRegression(X, y, x_dtype="fp32", kernel="mfma")
         e32, o32, w32, _, k32, n32 = run_form(m32, C, nw, args.fp32_steps, z0, keys,
-                                              state=post_state)
+                                              state=post_state, dense=args.dense_mass)
         me32, rh32 = min_ess(o32["z"]) if args.fp32_steps >= 4 else (float("nan"), float("nan"))
         # the bf16 chains' ESS over the same number of draws, for comparison
         mes_b, _ = min_ess(out["z"][:, :args.fp32_steps])
@@ -272,13 +277,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"logistic NUTS {args.config}: N={N}, D={D}, chains={C}, "
                                f"X {args.x_dtype}, max_tree_depth 10, target_accept 0.8, "
-                               f"diag mass", "global_batch": C, "seq_len": N,
+                               f"{'dense' if args.dense_mass else 'diag'} mass",
+                   "global_batch": C, "seq_len": N,
                    "parallelism": "chains",
                    "init_params": "Laplace mode + Laplace sd * uniform(-2, 2) per chain",
-                   "inverse_mass_matrix": "Laplace variances (diagonal)",
+                   "inverse_mass_matrix": ("Laplace covariance (dense_mass=True)"
+                                           if args.dense_mass else "Laplace variances (diagonal)"),
                    "find_heuristic_step_size": True},
         "valid": valid,
         "setup_s": setup_s,
+        "tuning": eng.tuning(),
         "min_ess_per_s": mess / wall,
         "min_ess": mess,
         "min_ess_per_iteration": mess / args.steps,
@@ -336,6 +344,7 @@ def bench_sv(args):
             "mean_accept_prob": float(out["accept_prob"].float().mean()),
             "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
             "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
+            "tuning": eng.tuning(),
             "hbm_bytes_per_grad_eval_per_chain": model.algorithmic_bytes(1)}
     log(json.dumps(line))
     print(json.dumps(line), flush=True)
@@ -374,6 +383,7 @@ def bench_hier(args):
             "mean_accept_prob": float(out["accept_prob"].float().mean()),
             "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
             "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
+            "tuning": eng.tuning(),
             "algorithmic_bytes_per_sweep": model.algorithmic_bytes(C),
             "hbm_GBps_at_grad_rate": model.algorithmic_bytes(C) * ge / C / wall / 1e9}
     log(json.dumps(line))

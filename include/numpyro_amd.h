@@ -30,7 +30,7 @@ extern "C" {
 
 #define NAX_X_F32 0
 #define NAX_X_BF16 1
-#define NAX_X_FP8 2 /* reserved: refused with NAX_EINVAL */
+#define NAX_X_FP8 2 /* OCP e4m3fn + power-of-two column scales, VALU tile kernel */
 
 typedef struct nax_sampler nax_sampler;
 
@@ -180,8 +180,10 @@ int nax_model_hier(nax_sampler* h, const float* d_x, const float* d_y, const int
  * read it transposed for X^T R with bf16 residuals; 2 keeps a second, fp16
  * copy for X^T R with fp16 residuals (twice the X bytes per evaluation, a
  * gradient rounding of 2^-11 instead of 2^-8 per term, and max|X| < 60000).
- * Other kernel values with bf16 storage, and 4 with fp32 storage, are
- * refused (NAX_EINVAL). */
+ * fp8 storage (NAX_X_FP8; the library quantises d_x as nax_fp8_quantize
+ * does and keeps the codes) runs on the VALU tile kernel: 0 (auto, row
+ * splits autotuned) or 1.  Other kernel values with bf16 or fp8 storage,
+ * and 4 with fp32 storage, are refused (NAX_EINVAL). */
 int nax_model_logistic_ex(nax_sampler* h, const float* d_x, const float* d_y, int64_t n,
                           int32_t d, double prior_scale, int32_t x_dtype, int32_t kernel);
 
@@ -260,6 +262,15 @@ int nax_set_positive_transform(nax_sampler* h, int32_t kind);
  * XLA program per model. */
 int nax_tuning(nax_sampler* h, char* buf, int32_t len);
 
+/* Fixes the launch shape the autotuner would pick: waves per chain (1, 2,
+ * 4, 8, 16) of the per-chain persistent models (stochastic volatility, the
+ * one-wave-per-chain logistic form), or chains per workgroup (a divisor of
+ * chains_per_group) of the hierarchical model.  Draws depend on the waves
+ * per chain (the order of its sums), not on the chains per workgroup; a
+ * fixed shape makes runs repeat bit for bit.  No counterpart in the
+ * reference. */
+int nax_set_launch_shape(nax_sampler* h, int32_t shape);
+
 /* HMCState / post_warmup_state access (mcmc.py:558-587): copies the
  * per-chain state out of / into the sampler. */
 int nax_get_state(nax_sampler* h, const nax_state_view* v);
@@ -271,6 +282,15 @@ int nax_set_state(nax_sampler* h, const nax_state_view* v);
  * them in hmc_state.adapt_state). */
 int nax_get_adapt_state(nax_sampler* h, const nax_adapt_view* v);
 int nax_set_adapt_state(nax_sampler* h, const nax_adapt_view* v);
+
+/* The FP8 storage of nax_model_logistic(_ex) with x_dtype NAX_X_FP8 (the
+ * fp8 variant of the C5 precision configuration; the reference computes
+ * jnp.dot(data, coefs), examples/covtype.py:70, in the data's dtype): per
+ * column d the power-of-two scale s_d >= max_n |x[n][d]| / 448 (1 for an
+ * all-zero column) into d_scales [d], and the round-to-nearest-even OCP
+ * e4m3fn byte of x[n][d] / s_d into d_codes [n][d].  The model's design is
+ * then x_q = e4m3(code) * s_d, exact in fp32. */
+int nax_fp8_quantize(const float* d_x, int64_t n, int32_t d, uint8_t* d_codes, float* d_scales);
 
 /* Potential and gradient of the built-in model at d_z [count, ld]:
  * potential_energy + value_and_grad (util.py:333-358, hmc_util.py:242-252). */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "nax_model_external": (C.c_int, [C.c_void_p]),
     "nax_set_positive_transform": (C.c_int, [C.c_void_p, C.c_int32]),
     "nax_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "nax_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nax_fp8_quantize": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "nax_model_logistic_ecs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                          C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                          C.c_void_p, C.c_void_p, C.c_void_p]),

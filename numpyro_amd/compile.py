@@ -545,6 +545,10 @@ class CompiledModel(Model):
         self.params = params
         self.site_sizes = tuple(params["sites"])
         self.dim = sum(size for _, size, _ in self.site_sizes)
+        # latent sites in the order the model samples them (the order
+        # find_valid_initial_params draws their initial values in)
+        self.site_order = tuple(k for k, v in tr.items()
+                                if v["type"] == "sample" and not v["is_observed"])
         self.fn, self.args, self.kwargs = fn, args, kwargs
         self.deterministic = _deterministic_names(tr)
         self.x_dtype, self.kernel = x_dtype, kernel

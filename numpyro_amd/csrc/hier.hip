@@ -18,7 +18,10 @@
 // its chain's NUTS state machine; when any chain of the block needs a
 // gradient the block evaluates all K together (chains that are already done
 // ride along and ignore the result).  Blocks take models heaviest first
-// (N_i D_i descending) to even out the tail of the launch.
+// (N_i D_i descending) to even out the tail of the launch.  A model's K
+// chains may also be split over K / KB workgroups of KB waves each (each
+// then reads X_i for its KB chains): more workgroups for the heavy models
+// against more passes over their X; the autotuner picks KB.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -85,8 +88,8 @@ struct HierPipe {
 // accesses of one wavefront complete in order), and buffer t & 1 is next
 // written (tile t + 2) after every wave has passed the barrier of tile t + 1.
 template <int K>
-__device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, int ldz,
-                                  float* gout, int ldg, HierShared<K>& sh) {
+__device__ double hier_group_eval(const HierData& hd, int i, int sub, const float* zreq,
+                                  int ldz, float* gout, int ldg, HierShared<K>& sh) {
   using P = HierPipe<K>;
   constexpr int NT = P::NT, EPT = P::EPT, PF = P::PF;
   const int w = threadIdx.x >> 6;
@@ -97,7 +100,7 @@ __device__ double hier_group_eval(const HierData& hd, int i, const float* zreq, 
   const int ntiles = (int)((N + kHierRows - 1) / kHierRows);
   const float* X = hd.x + hd.elt_off[i];
   const float* Y = hd.y + hd.row_off[i];
-  const int c = i * K + w;
+  const int c = i * hd.K + sub * K + w;
   const float* z = zreq + (size_t)c * ldz;
   float* g = gout + (size_t)c * ldg;
   const float mu = z[D];
@@ -220,24 +223,29 @@ __global__ void __launch_bounds__(64 * K)
 hier_eval_kernel(HierData hd, const float* __restrict__ zreq, int ldz, double* __restrict__ pe,
                  float* __restrict__ gout, int ldg) {
   __shared__ HierShared<K> sh;
-  const int i = hd.order[blockIdx.x];
-  const double u = hier_group_eval<K>(hd, i, zreq, ldz, gout, ldg, sh);
-  if ((threadIdx.x & 63) == 0) pe[i * K + (threadIdx.x >> 6)] = u;
+  const int per = hd.K / K;  // workgroups per model
+  const int i = hd.order[blockIdx.x / per];
+  const int sub = blockIdx.x % per;
+  const double u = hier_group_eval<K>(hd, i, sub, zreq, ldz, gout, ldg, sh);
+  if ((threadIdx.x & 63) == 0) pe[i * hd.K + sub * K + (threadIdx.x >> 6)] = u;
 }
 
 template <int K, bool DENSE>
 __global__ void __launch_bounds__(64 * K, 2)
-hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
-                       ChainState* __restrict__ states, float* __restrict__ vec,
-                       double* __restrict__ welf, float* __restrict__ zreq,
-                       float* __restrict__ gout, double* __restrict__ peout,
-                       int64_t* __restrict__ grad_evals) {
+hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col,
+                       const int64_t* __restrict__ d_target, ChainState* __restrict__ states,
+                       float* __restrict__ vec, double* __restrict__ welf,
+                       float* __restrict__ zreq, float* __restrict__ gout,
+                       double* __restrict__ peout, int64_t* __restrict__ grad_evals) {
   __shared__ HierShared<K> sh;
   __shared__ int need_any;
   __shared__ ChainState sS[K];
-  const int i = hd.order[blockIdx.x];
+  const int64_t target = *d_target;
+  const int per = hd.K / K;  // workgroups per model
+  const int i = hd.order[blockIdx.x / per];
+  const int sub = blockIdx.x % per;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int c = i * K + w;
+  const int c = i * hd.K + sub * K + w;
   const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
@@ -258,7 +266,7 @@ hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col, int64_t target,
     __threadfence_block();
     __syncthreads();
     if (!need_any) break;
-    const double u = hier_group_eval<K>(hd, i, zreq, cfg.Dp, gout, cfg.Dp, sh);
+    const double u = hier_group_eval<K>(hd, i, sub, zreq, cfg.Dp, gout, cfg.Dp, sh);
     if (need) pe = u;
     __threadfence_block();
   }
@@ -298,12 +306,28 @@ struct HierPotential : Potential {
   double alg_bytes(int) const override { return bytes; }
   double alg_flops(int) const override { return flops; }
 
+  int kb = 0;  // waves (chains) per workgroup, tuned; 0: hd.K
+  int tuned_chains = 0;
+  bool forced = false;
+  bool force_launch_shape(int shape) override {
+    if (shape < 1 || shape > hd.K || hd.K % shape) return false;
+    kb = shape;
+    forced = true;
+    tuning = "hier: " + std::to_string(kb) + " chains per workgroup of " +
+             std::to_string(hd.K) + " per model (fixed)";
+    return true;
+  }
+
+  int waves() const { return kb > 0 ? kb : hd.K; }
+
   hipError_t eval(hipStream_t s, const float* z, int ldz, int Cn, double* pe, float* g,
                   int ldg) override {
     if (Cn != C || ldz < maxd + 2 || ldg < maxd + 2) return hipErrorInvalidValue;
+    const int w = waves();
+    const dim3 grid(groups * (hd.K / w));
     mark_begin(s);
-    switch (hd.K) {
-#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_eval_kernel<KV>, dim3(groups), dim3(64 * KV), 0, s, hd, z, ldz, pe, g, ldg); break;
+    switch (w) {
+#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_eval_kernel<KV>, grid, dim3(64 * KV), 0, s, hd, z, ldz, pe, g, ldg); break;
       NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
 #undef NAX_K
       default: return hipErrorInvalidValue;
@@ -311,26 +335,56 @@ struct HierPotential : Potential {
     mark_end(s);
     return hipGetLastError();
   }
-  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
-                            int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
-                            float* gout, double* peout, int64_t* gevals) override {
-    mark_begin(s);
-    switch (hd.K) {
-#define NAX_K(KV)                                                                      \
-  case KV:                                                                             \
-    if (cfg.dense)                                                                     \
-      hipLaunchKernelGGL((hier_persistent_kernel<KV, true>), dim3(groups), dim3(64 * KV), 0, s, \
-                         hd, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);      \
-    else                                                                               \
-      hipLaunchKernelGGL((hier_persistent_kernel<KV, false>), dim3(groups), dim3(64 * KV), 0, \
-                         s, hd, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals);   \
+
+  hipError_t launch(int w, hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                    const int64_t* d_target, ChainState* st, float* vec, double* wel, float* zreq,
+                    float* gout, double* peout, int64_t* gevals) const {
+    const dim3 grid(groups * (hd.K / w));
+    switch (w) {
+#define NAX_K(KV)                                                                          \
+  case KV:                                                                                 \
+    if (cfg.dense)                                                                         \
+      hipLaunchKernelGGL((hier_persistent_kernel<KV, true>), grid, dim3(64 * KV), 0, s, hd, \
+                         cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals);     \
+    else                                                                                   \
+      hipLaunchKernelGGL((hier_persistent_kernel<KV, false>), grid, dim3(64 * KV), 0, s,    \
+                         hd, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals); \
     break;
       NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
 #undef NAX_K
       default: return hipErrorInvalidValue;
     }
-    mark_end(s);
     return hipGetLastError();
+  }
+
+  hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
+                                double* wel, float* zreq, float* gout, double* peout) override {
+    // chains per workgroup: K (one workgroup per model) or K/2, K/4 (each
+    // model's chains split over 2 or 4 workgroups), timed over a few
+    // iterations from a scratch copy of the state
+    if (forced || (kb > 0 && tuned_chains == cfg.C)) return hipSuccess;
+    std::vector<int> cands;
+    for (int w = hd.K; w >= 1 && (int)cands.size() < 3; w /= 2) cands.push_back(w);
+    std::string log;
+    auto go = [&](int w, hipStream_t ss, const Collect& col, const int64_t* tgt, ChainState* st2,
+                  float* vec2, double* wel2, float* zr2, float* g2, double* pe2, int64_t* ge2) {
+      return launch(w, ss, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2, ge2);
+    };
+    kb = tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, 3, cands, go, &log);
+    tuned_chains = cfg.C;
+    tuning = "hier: " + std::to_string(kb) + " chains per workgroup of " +
+             std::to_string(hd.K) + " per model (ms for 3 iterations:" + log + ")";
+    return hipSuccess;
+  }
+
+  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                            const int64_t* d_target, ChainState* st, float* vec, double* wel,
+                            float* zreq, float* gout, double* peout, int64_t* gevals) override {
+    mark_begin(s);
+    const hipError_t e = launch(waves(), s, cfg, col, d_target, st, vec, wel, zreq, gout, peout,
+                                gevals);
+    mark_end(s);
+    return e;
   }
 };
 

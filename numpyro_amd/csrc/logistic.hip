@@ -16,6 +16,7 @@
 // prior.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -30,12 +31,83 @@ __device__ __forceinline__ float softplusf_(float x) {
 }
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
-// ---- plain FP32 form ----------------------------------------------------------
+// ---- X storage forms of the tile kernel ----------------------------------------
+// fp32: X as given.  FP8: X[n][d] = e4m3(code[n][d]) * scale[d], the OCP
+// e4m3fn byte (1 sign, 4 exponent bits with bias 7, 3 mantissa bits; no
+// infinities) with a power-of-two scale per column; the value of a code
+// comes from a 256-entry table, so the products are exact in fp32.
+__constant__ float kE4M3[256];
+
+struct XF32 {
+  const float* x;
+  __device__ __forceinline__ float operator()(int64_t n, int d, int D) const {
+    return x[n * D + d];
+  }
+};
+struct XFP8 {
+  const uint8_t* q;
+  const float* scale;
+  __device__ __forceinline__ float operator()(int64_t n, int d, int D) const {
+    return kE4M3[q[n * D + d]] * scale[d];
+  }
+};
+
+// Round-to-nearest-even e4m3fn code of v with |v| <= 448 (subnormal step
+// 2^-9; codes count up through the binades, so a mantissa carry moves to
+// the next exponent by itself).
+__device__ __forceinline__ uint8_t e4m3_encode(float v) {
+  const uint8_t sign = v < 0.0f ? 0x80 : 0;
+  const float a = fabsf(v);
+  int code;
+  if (a < 0.015625f) {                       // below 2^-6: subnormal step 2^-9
+    code = (int)rintf(a * 512.0f);
+  } else {
+    int e;
+    frexpf(a, &e);                           // a = m 2^e, m in [0.5, 1)
+    e -= 1;                                  // leading bit 2^e, e in [-6, 8]
+    const int q = (int)rintf(ldexpf(a, 3 - e));   // 8..16, exact scaling
+    code = ((e + 7) << 3) + (q - 8);
+  }
+  return (uint8_t)(sign | (code > 0x7E ? 0x7E : code));
+}
+
+// Per-column scale: the power of two s >= max_n |X[n][d]| / 448 (1 for an
+// all-zero column), one workgroup per column.
+__global__ void __launch_bounds__(256)
+fp8_column_scales(const float* __restrict__ X, int64_t N, int D, float* __restrict__ scale) {
+  const int d = blockIdx.x;
+  float m = 0.0f;
+  for (int64_t n = threadIdx.x; n < N; n += 256) m = fmaxf(m, fabsf(X[n * D + d]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float s = 1.0f;
+    if (m > 0.0f) {
+      int e;
+      frexpf(m / 448.0f, &e);                // m / 448 = f 2^e, f in [0.5, 1)
+      s = ldexpf(1.0f, e);                   // 2^e >= m / 448
+      if (s * 0.5f >= m / 448.0f) s *= 0.5f;  // m / 448 a power of two itself
+    }
+    scale[d] = s;
+  }
+}
+
+__global__ void fp8_encode(const float* __restrict__ X, int64_t total, int D,
+                           const float* __restrict__ scale, uint8_t* __restrict__ q) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x)
+    q[i] = e4m3_encode(X[i] / scale[i % D]);  // division by a power of two: exact
+}
+
+// ---- the VALU tile kernel ---------------------------------------------------------
 // Block: 256 threads, BN rows x BC chains per step.  DCH = ceil(D/64) column
 // chunks held per thread in the gradient accumulator.
-template <int BN, int BC, int DCH>
+template <int BN, int BC, int DCH, class XL = XF32>
 __global__ void __launch_bounds__(256)
-logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int64_t N, int D,
+logit_partial_f32(XL X, const float* __restrict__ y, int64_t N, int D,
                   const float* __restrict__ Z, int ldz, int C, int RB,
                   float* __restrict__ wsg, double* __restrict__ wsu) {
   extern __shared__ float smem[];
@@ -71,7 +143,7 @@ logit_partial_f32(const float* __restrict__ X, const float* __restrict__ y, int6
     for (int i = t; i < BN * D; i += 256) {
       int r = i / D, d = i % D;
       int64_t n = n0 + r;
-      Xs[r * DP + d] = (n < N) ? X[n * D + d] : 0.0f;
+      Xs[r * DP + d] = (n < N) ? X(n, d, D) : 0.0f;
     }
     __syncthreads();
     // L = X_tile Z^T, R = sigmoid(L) - y, U += softplus(L) - y L
@@ -179,6 +251,8 @@ struct LogisticPotential : Potential {
   int d = 0;
   double prior_scale = 1.0;
   int x_dtype = NAX_X_F32;
+  uint8_t* q8 = nullptr;     // FP8 form: e4m3 codes [n][d]
+  float* scale8 = nullptr;   // and the per-column scales [d]
   float* wsg = nullptr;
   double* wsu = nullptr;
   size_t ws_chains = 0;
@@ -188,10 +262,16 @@ struct LogisticPotential : Potential {
   ~LogisticPotential() override {
     hipFree(wsg);
     hipFree(wsu);
+    hipFree(q8);
+    hipFree(scale8);
   }
   int dim() const override { return d; }
-  const char* name() const override { return "logit_partial_f32"; }
-  double alg_bytes(int C) const override { return 4.0 * (double)n * d + 4.0 * n; }
+  const char* name() const override {
+    return x_dtype == NAX_X_FP8 ? "logit_partial_fp8" : "logit_partial_f32";
+  }
+  double alg_bytes(int C) const override {
+    return (x_dtype == NAX_X_FP8 ? 1.0 : 4.0) * (double)n * d + 4.0 * n;
+  }
   double alg_flops(int C) const override { return 4.0 * (double)n * d * C; }
 
   hipError_t ensure_ws(int C, int RB) {
@@ -223,9 +303,13 @@ struct LogisticPotential : Potential {
     size_t shm = sizeof(float) * ((size_t)BC * DP + (size_t)BN * DP + BN * BC) + 256 * sizeof(double);
     dim3 grid(RB, ctiles);
     mark_begin(s);
-#define NAX_LAUNCH(DC)                                                                    \
-  hipLaunchKernelGGL((logit_partial_f32<BN, BC, DC>), grid, dim3(256), shm, s, x, y, n, d, z, \
-                     ldz, C, RB, wsg, wsu)
+#define NAX_LAUNCH(DC)                                                                       \
+  if (x_dtype == NAX_X_FP8)                                                                  \
+    hipLaunchKernelGGL((logit_partial_f32<BN, BC, DC, XFP8>), grid, dim3(256), shm, s,        \
+                       XFP8{q8, scale8}, y, n, d, z, ldz, C, RB, wsg, wsu);                   \
+  else                                                                                       \
+    hipLaunchKernelGGL((logit_partial_f32<BN, BC, DC, XF32>), grid, dim3(256), shm, s,        \
+                       XF32{x}, y, n, d, z, ldz, C, RB, wsg, wsu)
     switch (dch) {
       case 1: NAX_LAUNCH(1); break;
       case 2: NAX_LAUNCH(2); break;
@@ -249,6 +333,45 @@ struct LogisticPotential : Potential {
   }
 };
 
+// The e4m3fn value of every code (OCP FP8: no infinities, S.1111.111 NaN).
+static void fill_e4m3_table(float* t) {
+  for (int c = 0; c < 256; ++c) {
+    const int e = (c >> 3) & 15, m = c & 7;
+    float v;
+    if (e == 15 && m == 7) v = NAN;
+    else if (e == 0) v = std::ldexp((float)m, -9);
+    else v = std::ldexp(1.0f + m / 8.0f, e - 7);
+    t[c] = (c & 0x80) ? -v : v;
+  }
+}
+
+// X [n][d] fp32 -> codes [n][d] (allocated here) and power-of-two column
+// scales [d]; uploads the decoding table.  0 or an error code (err set).
+int fp8_quantize(const float* x, int64_t n, int d, uint8_t** q, float** scale, hipStream_t s,
+                 std::string* err) {
+  float table[256];
+  fill_e4m3_table(table);
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kE4M3), table, sizeof(table));
+  if (e == hipSuccess && !*q) e = hipMalloc(q, (size_t)n * d);
+  if (e == hipSuccess && !*scale) e = hipMalloc(scale, sizeof(float) * d);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fp8_column_scales, dim3(d), dim3(256), 0, s, x, n, d, *scale);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    const int64_t total = n * (int64_t)d;
+    const int blocks = (int)std::min<int64_t>(4096, (total + 255) / 256);
+    hipLaunchKernelGGL(fp8_encode, dim3(blocks), dim3(256), 0, s, x, total, d, *scale, *q);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    *err = std::string("fp8 quantisation: ") + hipGetErrorString(e);
+    return NAX_EHIP;
+  }
+  return NAX_OK;
+}
+
 Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
                               double prior_scale, bool bf16, bool single, hipStream_t s,
                               std::string* err);
@@ -271,16 +394,22 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
     *err = "kernel 4 (single-copy bf16) needs bf16 storage";
     return nullptr;
   }
-  if (x_dtype != NAX_X_F32) {
+  if (x_dtype == NAX_X_FP8 && fuse_variant != 0 && fuse_variant != 1) {
+    *err = "fp8 storage runs on the VALU tile kernel: kernel 0 (auto) or 1";
+    return nullptr;
+  }
+  if (x_dtype != NAX_X_F32 && x_dtype != NAX_X_FP8) {
     *err = "x_dtype not supported";
     return nullptr;
   }
   // a handful of chains on a small design: one wavefront per chain, the
   // whole sampling loop in one persistent launch
-  if (fuse_variant == 3 ||
-      (fuse_variant == 0 && num_chains <= 16 && d <= 64 && (double)n * d <= (double)(1 << 20)))
+  const bool fp8 = x_dtype == NAX_X_FP8;
+  if (!fp8 && (fuse_variant == 3 || (fuse_variant == 0 && num_chains <= 16 && d <= 64 &&
+                                     (double)n * d <= (double)(1 << 20))))
     return make_logistic_chain(x, y, n, d, prior_scale, err);
-  const bool mfma = fuse_variant == 2 || (fuse_variant == 0 && num_chains >= 32 && d <= 256);
+  const bool mfma = !fp8 && (fuse_variant == 2 ||
+                             (fuse_variant == 0 && num_chains >= 32 && d <= 256));
   if (mfma && !(fuse_variant == 0 && num_chains <= 128 && d <= 320))
     return make_logistic_mfma(x, y, n, d, prior_scale, false, false, s, err);
   if (d > 320) {
@@ -294,6 +423,15 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
   p->d = d;
   p->prior_scale = prior_scale;
   p->x_dtype = x_dtype;
+  if (fp8) {
+    // the library's own e4m3 copy of X with power-of-two column scales
+    p->x = nullptr;
+    const int qrc = fp8_quantize(x, n, d, &p->q8, &p->scale8, s, err);
+    if (qrc) {
+      delete p;
+      return nullptr;
+    }
+  }
   if (fuse_variant != 0 || num_chains < 1) return p;
   // Autotuner ("auto" kernel): row splits of the tile kernel, and for 32 <=
   // C <= 128 the tile kernel against the fp32 MFMA kernel, each timed over a
@@ -349,3 +487,16 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
 }
 
 }  // namespace nax
+
+extern "C" int nax_fp8_quantize(const float* d_x, int64_t n, int32_t d, uint8_t* d_codes,
+                                float* d_scales) {
+  using namespace nax;
+  if (!d_x || !d_codes || !d_scales || n < 1 || d < 1) {
+    set_error("nax_fp8_quantize: need x, codes, scales, n >= 1, d >= 1");
+    return NAX_EINVAL;
+  }
+  std::string err;
+  const int rc = fp8_quantize(d_x, n, d, &d_codes, &d_scales, nullptr, &err);
+  if (rc) set_error(err);
+  return rc;
+}

@@ -123,27 +123,51 @@ struct LogisticChainPotential : Potential {
     mark_end(s);
     return hipGetLastError();
   }
-  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
-                            int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
-                            float* gout, double* peout, int64_t* gevals) override {
+  hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
+                                double* wel, float* zreq, float* gout, double* peout) override {
     // one chain per workgroup; the waves per chain (rows of X split over
     // 64 wpc threads) are picked on the first launch by timing a few
     // iterations of each candidate from a scratch copy of the state
-    using Mdl = LogisticChainModel<DMAX>;
-    if (wpc == 0 || tuned_chains != cfg.C) {
-      std::string log;
-      wpc = tune_chain_groups<Mdl, false>(s, m, cfg, st, vec, wel, zreq, gout, peout, 4, &log);
+    if (!forced && (wpc == 0 || tuned_chains != cfg.C)) {
+      const std::string key = "logistic_chain n=" + std::to_string(m.n) + " d=" +
+                              std::to_string(m.d) + " C=" + std::to_string(cfg.C) +
+                              " dense=" + std::to_string(cfg.dense);
       tuned_chains = cfg.C;
+      if (tuned_shape(key, &wpc, false)) {
+        tuning = "logistic_chain: " + std::to_string(wpc) + " waves per chain (tuned earlier "
+                 "for " + key + ")";
+        return hipSuccess;
+      }
+      std::string log;
+      wpc = tune_chain_groups<LogisticChainModel<DMAX>, false>(s, m, cfg, st, vec, wel, zreq,
+                                                               gout, peout, 4, &log);
+      tuned_shape(key, &wpc, true);
       tuning = "logistic_chain: " + std::to_string(wpc) + " waves per chain (ms for 4 "
                "iterations:" + log + ")";
     }
+    return hipSuccess;
+  }
+  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                            const int64_t* d_target, ChainState* st, float* vec, double* wel,
+                            float* zreq, float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    hipError_t e = launch_chain_groups<Mdl, false>(wpc, s, m, cfg, col, target, st, vec, wel,
-                                                   zreq, gout, peout, gevals);
+    hipError_t e = launch_chain_groups<LogisticChainModel<DMAX>, false>(
+        wpc > 0 ? wpc : 1, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals);
     mark_end(s);
     return e;
   }
+  bool force_launch_shape(int shape) override {
+    for (int w : kGroupSizes)
+      if (w == shape) {
+        wpc = shape;
+        forced = true;
+        tuning = "logistic_chain: " + std::to_string(wpc) + " waves per chain (fixed)";
+        return true;
+      }
+    return false;
+  }
   int wpc = 0, tuned_chains = 0;
+  bool forced = false;
 };
 
 // d <= 64; x and y are the caller's device arrays (kept, not copied).

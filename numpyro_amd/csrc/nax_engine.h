@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -928,11 +930,14 @@ inline int persistent_lds_slots(int Dp, int nvec, int wpc = 1) {
 // back.
 template <class M, bool DENSE, int WPB, bool LDSV = false, int WPC = 1>
 __global__ void __launch_bounds__(64 * WPB * WPC)
-persistent_kernel(M m, EngineCfg cfg, Collect col, int64_t target,
+persistent_kernel(M m, EngineCfg cfg, Collect col, const int64_t* __restrict__ d_target,
                   ChainState* __restrict__ states, float* __restrict__ vec,
                   double* __restrict__ welf, float* __restrict__ zreq, float* __restrict__ gout,
                   double* __restrict__ peout, int64_t* __restrict__ grad_evals, int nl) {
   static_assert(WPC == 1 || WPB == 1, "a chain group fills its workgroup");
+  // the iteration target sits in device memory, so that one captured graph
+  // (target step + this launch) replays for every chunk of iterations
+  const int64_t target = *d_target;
   __shared__ ChainState sS[WPB * WPC];
   __shared__ double nax_red[WPC > 1 ? kRedSlots * WPC : 1];
   __shared__ float nax_redf[WPC > 1 ? NAX_WAVE * WPC : 1];
@@ -983,27 +988,54 @@ namespace nax {
 // Waves per chain the persistent launchers instantiate.
 constexpr int kGroupSizes[] = {1, 2, 4, 8, 16};
 
+// Launch shapes tuned so far in this process, keyed by the model's shape:
+// every sampler of one shape then runs the same launch (and the same draws
+// for the same keys), and pays for the tuning once.
+inline bool tuned_shape(const std::string& key, int* value, bool store) {
+  static std::mutex mu;
+  static std::map<std::string, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  if (store) {
+    cache[key] = *value;
+    return true;
+  }
+  auto it = cache.find(key);
+  if (it == cache.end()) return false;
+  *value = it->second;
+  return true;
+}
+
+// Advances the device-resident iteration target of persistent launches:
+// *t = min(final, *t + step) (one thread).
+static __global__ void step_target_kernel(int64_t* t, int64_t final, int64_t step) {
+  const int64_t v = *t + step;
+  *t = v < final ? v : final;
+}
+static __global__ void set_target_kernel(int64_t* t, int64_t v) { *t = v; }
+
 // Launches persistent_kernel<M, ., 1, LDSV, wpc> (one chain per workgroup),
 // with the vector slots in LDS when LDSV.  Dense mass runs on one wave.
 template <class M, bool LDSV>
 hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineCfg& cfg,
-                               const Collect& col, int64_t target, ChainState* st, float* vec,
-                               double* wel, float* zreq, float* gout, double* peout,
-                               int64_t* gevals) {
+                               const Collect& col, const int64_t* d_target, ChainState* st,
+                               float* vec, double* wel, float* zreq, float* gout,
+                               double* peout, int64_t* gevals) {
   if (cfg.dense) wpc = 1;
 #define NAX_GROUP_LAUNCH(DN, W)                                                             \
   {                                                                                         \
     const int nl = LDSV ? persistent_lds_slots(cfg.Dp, cfg.nvec, W) : 0;                    \
     const size_t lds = (size_t)nl * cfg.Dp * sizeof(float);                                 \
-    if (LDSV) {                                                                             \
+    static size_t lds_set = 0; /* set once per size (also outside graph capture) */        \
+    if (LDSV && lds > lds_set) {                                                            \
       hipError_t e = hipFuncSetAttribute(                                                   \
           (const void*)persistent_kernel<M, DN, 1, LDSV, W>,                                \
           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                            \
       if (e != hipSuccess) return e;                                                        \
+      lds_set = lds;                                                                        \
     }                                                                                       \
     hipLaunchKernelGGL((persistent_kernel<M, DN, 1, LDSV, W>), dim3(cfg.C), dim3(64 * W),   \
-                       lds, s, m, cfg, col, target, st, vec, wel, zreq, gout, peout, gevals, \
-                       nl);                                                                 \
+                       lds, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout,      \
+                       gevals, nl);                                                         \
     return hipGetLastError();                                                               \
   }
   if (cfg.dense) NAX_GROUP_LAUNCH(true, 1)
@@ -1018,16 +1050,17 @@ hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineC
 #undef NAX_GROUP_LAUNCH
 }
 
-// Autotuner of the waves per chain: from a scratch copy of the current
-// sampler state, each candidate runs the real persistent kernel for `iters`
-// iterations (the same NUTS work, nothing collected) timed with HIP events;
-// the fastest is returned.  The sampler's own state is not touched.  `log`
-// receives "wpc:ms" for every candidate.
-template <class M, bool LDSV>
-int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const ChainState* st,
-                      const float* vec, const double* wel, const float* zreq, const float* gout,
-                      const double* peout, int iters, std::string* log) {
-  if (cfg.dense) return 1;
+// Autotuner of a persistent launch parameter: from a scratch copy of the
+// current sampler state, each candidate runs the real persistent kernel
+// (launch(cand, stream, Collect, d_target, st, vec, wel, zreq, gout, peout,
+// gevals)) for `iters` iterations -- the same NUTS work, nothing collected --
+// timed with HIP events; the fastest candidate is returned.  The sampler's
+// own state is not touched.  `log` receives "candidate:ms" for each.
+template <class Launch>
+int tune_persistent(hipStream_t s, const EngineCfg& cfg, const ChainState* st, const float* vec,
+                    const double* wel, const float* zreq, const float* gout, const double* peout,
+                    int iters, const std::vector<int>& cands, Launch launch, std::string* log) {
+  if (cands.size() < 2) return cands.empty() ? 1 : cands[0];
   const size_t nst = sizeof(ChainState) * cfg.C;
   const size_t nvec = sizeof(float) * (size_t)cfg.C * cfg.nvec * cfg.Dp;
   const size_t nwel = sizeof(double) * (size_t)cfg.C * 3 * cfg.Dp;
@@ -1036,34 +1069,35 @@ int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const Cha
   ChainState* st2 = nullptr;
   float *vec2 = nullptr, *zr2 = nullptr, *g2 = nullptr;
   double *wel2 = nullptr, *pe2 = nullptr;
-  int64_t* ge2 = nullptr;
+  int64_t* ge2 = nullptr;  // [0]: grad evals, [1]: the target
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::vector<ChainState> hs(cfg.C);
   bool ok = hipMalloc(&st2, nst) == hipSuccess && hipMalloc(&vec2, nvec) == hipSuccess &&
             hipMalloc(&wel2, nwel) == hipSuccess && hipMalloc(&zr2, nzg) == hipSuccess &&
             hipMalloc(&g2, nzg) == hipSuccess && hipMalloc(&pe2, npe) == hipSuccess &&
-            hipMalloc(&ge2, sizeof(int64_t)) == hipSuccess &&
+            hipMalloc(&ge2, 2 * sizeof(int64_t)) == hipSuccess &&
             hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
             hipMemcpyAsync(hs.data(), st, nst, hipMemcpyDeviceToHost, s) == hipSuccess &&
             hipStreamSynchronize(s) == hipSuccess;
-  int best = 1;
+  int best = cands[0];
   if (ok) {
     int64_t i0 = 0;
     for (const ChainState& c : hs) i0 = std::max<int64_t>(i0, c.i);
     Collect none;
     memset(&none, 0, sizeof(none));
     float best_ms = 1e30f;
-    for (int w : kGroupSizes) {
+    for (int w : cands) {
       float ms = 1e30f;
-      if (hipMemcpyAsync(st2, st, nst, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+      hipLaunchKernelGGL(set_target_kernel, dim3(1), dim3(1), 0, s, ge2 + 1, i0 + iters);
+      if (hipGetLastError() == hipSuccess &&
+          hipMemcpyAsync(st2, st, nst, hipMemcpyDeviceToDevice, s) == hipSuccess &&
           hipMemcpyAsync(vec2, vec, nvec, hipMemcpyDeviceToDevice, s) == hipSuccess &&
           hipMemcpyAsync(wel2, wel, nwel, hipMemcpyDeviceToDevice, s) == hipSuccess &&
           hipMemcpyAsync(zr2, zreq, nzg, hipMemcpyDeviceToDevice, s) == hipSuccess &&
           hipMemcpyAsync(g2, gout, nzg, hipMemcpyDeviceToDevice, s) == hipSuccess &&
           hipMemcpyAsync(pe2, peout, npe, hipMemcpyDeviceToDevice, s) == hipSuccess &&
           hipEventRecord(e0, s) == hipSuccess &&
-          launch_chain_groups<M, LDSV>(w, s, m, cfg, none, i0 + iters, st2, vec2, wel2, zr2, g2,
-                                       pe2, ge2) == hipSuccess &&
+          launch(w, s, none, ge2 + 1, st2, vec2, wel2, zr2, g2, pe2, ge2) == hipSuccess &&
           hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess)
         hipEventElapsedTime(&ms, e0, e1);
       if (log) *log += " " + std::to_string(w) + ":" + std::to_string(ms);
@@ -1079,6 +1113,23 @@ int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const Cha
   if (e1) hipEventDestroy(e1);
   (void)hipGetLastError();  // a failed candidate leaves no sticky error behind
   return best;
+}
+
+// Waves per chain of a persistent per-chain model (kGroupSizes, one wave
+// under dense mass), by tune_persistent.
+template <class M, bool LDSV>
+int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const ChainState* st,
+                      const float* vec, const double* wel, const float* zreq, const float* gout,
+                      const double* peout, int iters, std::string* log) {
+  if (cfg.dense) return 1;
+  std::vector<int> cands(std::begin(kGroupSizes), std::end(kGroupSizes));
+  auto launch = [&](int w, hipStream_t ss, const Collect& col, const int64_t* tgt,
+                    ChainState* st2, float* vec2, double* wel2, float* zr2, float* g2,
+                    double* pe2, int64_t* ge2) {
+    return launch_chain_groups<M, LDSV>(w, ss, m, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2,
+                                        ge2);
+  };
+  return tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, iters, cands, launch, log);
 }
 
 }  // namespace nax

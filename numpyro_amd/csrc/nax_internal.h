@@ -98,6 +98,11 @@ struct Potential {
   virtual const char* name() const { return "potential"; }
   // Launch shapes the potential's autotuner chose (empty when it has none).
   std::string tuning;
+  // Fixes the tuned launch shape instead of timing candidates (waves per
+  // chain of per-chain persistent models, chains per workgroup of the
+  // hierarchical model); false when the model has no such shape or the
+  // value is not one of its candidates.
+  virtual bool force_launch_shape(int shape) { return false; }
   // Positive latent sites by ExpTransform (kind 0) or SoftplusTransform
   // (kind 1); false for a model without positive sites.
   virtual bool set_positive_transform(int kind) { return false; }
@@ -106,10 +111,20 @@ struct Potential {
   // Models whose potential needs no data shared across chains run the whole
   // sampling loop in one persistent launch per call.
   virtual bool persistent() const { return false; }
+  // d_target: device int64, the iteration every chain runs to.  A launch
+  // enqueues work only (no allocation, no synchronisation), so that it can
+  // be captured in a graph; prepare_persistent runs first, uncaptured
+  // (autotuning, workspaces).
   virtual hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
-                                    int64_t target, ChainState* st, float* vec, double* wel,
-                                    float* zreq, float* gout, double* peout, int64_t* gevals) {
+                                    const int64_t* d_target, ChainState* st, float* vec,
+                                    double* wel, float* zreq, float* gout, double* peout,
+                                    int64_t* gevals) {
     return hipErrorNotSupported;
+  }
+  virtual hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st,
+                                        float* vec, double* wel, float* zreq, float* gout,
+                                        double* peout) {
+    return hipSuccess;
   }
   // HMCECS models: one Gibbs update of every chain's subsample at its
   // current position (z, stride ldv), accepting into st[c].pe and the state

@@ -376,7 +376,7 @@ extern "C" int nax_create(const nax_config* c, nax_sampler** out) {
   NAX_HIP(hipMalloc(&h->peout, C * sizeof(double)));
   NAX_HIP(hipMalloc(&h->ctr, kBatch * sizeof(int)));
   NAX_HIP(hipMalloc(&h->active, C * sizeof(int)));
-  NAX_HIP(hipMalloc(&h->gevals, sizeof(int64_t)));
+  NAX_HIP(hipMalloc(&h->gevals, 2 * sizeof(int64_t)));  // grad evals, persistent target
   NAX_HIP(hipHostMalloc(&h->ctr_host, sizeof(int) * 2, hipHostMallocDefault));
   NAX_HIP(hipHostMalloc(&h->gevals_host, sizeof(int64_t), hipHostMallocDefault));
   NAX_HIP(hipMemsetAsync(h->vec, 0, C * e.nvec * Dp * sizeof(float), h->stream));
@@ -510,6 +510,18 @@ extern "C" int nax_model_external(nax_sampler* h) {
   delete h->pot;
   h->pot = nullptr;
   h->external = true;
+  return NAX_OK;
+}
+
+extern "C" int nax_set_launch_shape(nax_sampler* h, int32_t shape) {
+  if (!h) { set_error("null handle"); return NAX_EINVAL; }
+  if (!h->pot) { set_error("no built-in model set"); return NAX_ESTATE; }
+  if (!h->pot->force_launch_shape(shape)) {
+    set_error("nax_set_launch_shape: not a launch shape of this model (waves per chain 1, 2, 4, "
+              "8, 16 for per-chain persistent models; a divisor of chains_per_group for the "
+              "hierarchical model)");
+    return NAX_EINVAL;
+  }
   return NAX_OK;
 }
 
@@ -665,27 +677,69 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
   NAX_HIP(hipMemsetAsync(h->gevals, 0, sizeof(int64_t), h->stream));
   if (h->pot->persistent()) {
     // Whole loop on the device, in launches of at most kPersistIters
-    // iterations so that progress can be reported between them.
+    // iterations so that progress can be reported between them.  The
+    // iteration target lives in device memory (gevals[1]): each chunk is a
+    // target step kernel plus the persistent launch, enqueued directly or
+    // replayed as one hipGraph captured on the first chunk (the model's
+    // autotuning runs before, uncaptured).  Kernel timing brackets the main
+    // kernel with events, so it runs uncaptured.
     constexpr int64_t kPersistIters = 50;
     std::vector<ChainState> probe(1);
     NAX_HIP(hipMemcpyAsync(probe.data(), h->st, sizeof(ChainState), hipMemcpyDeviceToHost, h->stream));
     NAX_HIP(hipStreamSynchronize(h->stream));
+    int64_t* d_target = h->gevals + 1;
+    NAX_HIP(h->pot->prepare_persistent(h->stream, h->cfg, h->st, h->vec, h->wel, h->zreq,
+                                       h->gout, h->peout));
     int64_t t = probe[0].i;
+    hipLaunchKernelGGL(set_target_kernel, dim3(1), dim3(1), 0, h->stream, d_target, t);
+    NAX_HIP(hipGetLastError());
+    auto chunk = [&]() -> hipError_t {
+      hipLaunchKernelGGL(step_target_kernel, dim3(1), dim3(1), 0, h->stream, d_target, target,
+                         kPersistIters);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      return h->pot->run_persistent(h->stream, h->cfg, col, d_target, h->st, h->vec, h->wel,
+                                    h->zreq, h->gout, h->peout, h->gevals);
+    };
+    const bool use_graph = h->graphs && !h->timing;
+    hipGraphExec_t exec = nullptr;
+    struct ExecGuard {
+      hipGraphExec_t* e;
+      ~ExecGuard() { if (*e) hipGraphExecDestroy(*e); }
+    } guard{&exec};
     int64_t launches = 0;
-    do {
+    while (t < target) {
       t = std::min<int64_t>(target, t + kPersistIters);
       const int k = (int)(launches % kBatch);
-      int rc = eval_events(h, k);
-      if (rc) return rc;
-      NAX_HIP(h->pot->run_persistent(h->stream, h->cfg, col, t, h->st, h->vec, h->wel, h->zreq,
-                                     h->gout, h->peout, h->gevals));
-      h->pot->ev_begin = h->pot->ev_end = nullptr;
+      if (use_graph) {
+        if (!exec) {
+          NAX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+          const hipError_t le = chunk();
+          hipGraph_t graph = nullptr;
+          const hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+          if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            NAX_HIP(le != hipSuccess ? le : ce);
+          }
+          const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+          hipGraphDestroy(graph);
+          NAX_HIP(ie);
+        }
+        NAX_HIP(hipGraphLaunch(exec, h->stream));
+      } else {
+        int rc = eval_events(h, k);
+        if (rc) return rc;
+        NAX_HIP(chunk());
+        h->pot->ev_begin = h->pot->ev_end = nullptr;
+      }
       NAX_HIP(hipStreamSynchronize(h->stream));
-      rc = collect_timing(h, 1, k);
-      if (rc) return rc;
+      if (!use_graph) {
+        const int rc = collect_timing(h, 1, k);
+        if (rc) return rc;
+      }
       ++launches;
       if (h->progress_cb) h->progress_cb(h->progress_user, launches, h->cfg.C);
-    } while (t < target);
+    }
     if (gevals) {
       NAX_HIP(hipMemcpyAsync(h->gevals_host, h->gevals, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
       NAX_HIP(hipStreamSynchronize(h->stream));

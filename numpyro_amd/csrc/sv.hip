@@ -214,27 +214,51 @@ struct SVPotential : Potential {
     mark_end(s);
     return hipGetLastError();
   }
-  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
-                            int64_t target, ChainState* st, float* vec, double* wel, float* zreq,
-                            float* gout, double* peout, int64_t* gevals) override {
+  hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
+                                double* wel, float* zreq, float* gout, double* peout) override {
     // one chain per workgroup, its vector slots in LDS; the waves per chain
     // (a chain's time steps and vector passes split over 64 wpc threads)
     // are picked on the first launch by timing a few iterations of each
     // candidate from a scratch copy of the state
-    if (wpc == 0 || tuned_chains != cfg.C) {
+    if (!forced && (wpc == 0 || tuned_chains != cfg.C)) {
+      const std::string key = "sv T=" + std::to_string(m.T) + " nc=" +
+                              std::to_string(m.noncentred) + " C=" + std::to_string(cfg.C) +
+                              " dense=" + std::to_string(cfg.dense);
+      tuned_chains = cfg.C;
+      if (tuned_shape(key, &wpc, false)) {
+        tuning = "sv: " + std::to_string(wpc) + " waves per chain (tuned earlier for " + key +
+                 ")";
+        return hipSuccess;
+      }
       std::string log;
       wpc = tune_chain_groups<SVModel, true>(s, m, cfg, st, vec, wel, zreq, gout, peout, 2, &log);
-      tuned_chains = cfg.C;
+      tuned_shape(key, &wpc, true);
       tuning = "sv: " + std::to_string(wpc) + " waves per chain (ms for 2 iterations:" + log +
                ")";
     }
+    return hipSuccess;
+  }
+  hipError_t run_persistent(hipStream_t s, const EngineCfg& cfg, const Collect& col,
+                            const int64_t* d_target, ChainState* st, float* vec, double* wel,
+                            float* zreq, float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    hipError_t e = launch_chain_groups<SVModel, true>(wpc, s, m, cfg, col, target, st, vec, wel,
-                                                     zreq, gout, peout, gevals);
+    hipError_t e = launch_chain_groups<SVModel, true>(wpc > 0 ? wpc : 1, s, m, cfg, col, d_target,
+                                                     st, vec, wel, zreq, gout, peout, gevals);
     mark_end(s);
     return e;
   }
+  bool force_launch_shape(int shape) override {
+    for (int w : kGroupSizes)
+      if (w == shape) {
+        wpc = shape;
+        forced = true;
+        tuning = "sv: " + std::to_string(wpc) + " waves per chain (fixed)";
+        return true;
+      }
+    return false;
+  }
   int wpc = 0, tuned_chains = 0;
+  bool forced = false;
 };
 
 Potential* make_sv(const float* returns, int T, int noncentred, double sigma_rate, double nu_rate,

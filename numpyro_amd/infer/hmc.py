@@ -230,6 +230,12 @@ class Engine:
         _lib.check(self.lib.nax_set_progress(self.h, self._progress, None, int(every_ticks)),
                    "progress")
 
+    def set_launch_shape(self, shape):
+        """Fixes the launch shape the autotuner would pick (waves per chain of
+        per-chain persistent models, chains per workgroup of the
+        hierarchical model)."""
+        _lib.check(self.lib.nax_set_launch_shape(self.h, int(shape)), "nax_set_launch_shape")
+
     def tuning(self):
         """The launch shapes the model's autotuner chose, as text."""
         buf = C.create_string_buffer(1024)
@@ -505,16 +511,17 @@ class HMC(MCMCKernel):
             if isinstance(init_params, dict):
                 init_params = target.flatten(init_params)
         user_params = init_params is not None
+        init_values = None
         if init_params is None:
             if not isinstance(target, Model):
                 raise ValueError(
                     "Valid value of `init_params` must be provided with `potential_fn`.")
-            init_params = init_to_uniform(keys, target.dim, self_split=True)
             if self._init_strategy is not None:
-                if not hasattr(self._init_strategy, "apply"):
+                if not hasattr(self._init_strategy, "unconstrained"):
                     raise ValueError("init_strategy must be init_to_value(values=...) or "
                                      "init_to_feasible() from numpyro_amd.infer")
-                init_params = self._init_strategy.apply(init_params, target.site_sizes)
+                init_values = self._init_strategy.unconstrained(target.site_sizes)
+            init_params = torch.zeros((num_chains, target.dim))  # drawn below
         z0 = torch.as_tensor(init_params, dtype=torch.float32)
         if z0.ndim <= 1 and num_chains == 1:
             z0 = z0.reshape(1, -1)
@@ -534,7 +541,10 @@ class HMC(MCMCKernel):
             raise RuntimeError("Cannot find valid initial parameters. Please check your model "
                                "again.")
         if not user_params and not isinstance(model, PotentialFn):
-            z0 = _find_valid_initial_params(model, z0, keys, eng)
+            if init_values == "feasible":  # init_to_feasible: every coordinate 0
+                z0 = torch.zeros((num_chains, model.dim), dtype=torch.float32)
+            else:
+                z0 = _initial_params(model, keys, values=init_values, eng=eng)
         eng.init(z0, init_keys, self._inverse_mass_matrix)
         self._engine = eng
         if isinstance(model, Model) and not isinstance(model, PotentialFn):
@@ -590,13 +600,77 @@ def _split_first(keys):
     return out
 
 
-def _find_valid_initial_params(model, z0, keys, eng, max_tries=100):
-    """``find_valid_initial_params`` (``numpyro/infer/util.py:366-508``):
-    redraw a chain's initial point (uniform(-2, 2) from a fresh split of its
-    key) while U or dU/dz is not finite there, at most 100 times."""
-    z = z0.clone()
-    keys = np.array(keys, dtype=np.uint32).reshape(-1, 2)
-    C_, D = z.shape
+def _site_layout(model):
+    """The latent sites in the model's execution order, each with its
+    offset and size in the flat (sorted-name) latent and its transform."""
+    sizes = tuple(getattr(model, "site_sizes", ()) or ())
+    if not sizes:  # one unnamed block (ragged models): the whole latent
+        return [("_latent", 0, int(model.dim), None)]
+    offs, o = {}, 0
+    for name, size, tf in sizes:
+        offs[name] = (o, int(size), tf)
+        o += int(size)
+    order = tuple(getattr(model, "site_order", ()) or ()) or tuple(n for n, _, _ in sizes)
+    return [(name,) + offs[name] for name in order]
+
+
+class _InitKeys:
+    """find_valid_initial_params for init_to_uniform / init_to_value
+    (numpyro/infer/util.py:405-466, its prototype branch): per chain, the
+    key handed to initialize_model is the second half of split(chain key)
+    (HMC.init, hmc.py:744-750); every attempt draws ``key, subkey =
+    split(key)`` and then, site by site in the model's execution order,
+    ``uniform(subkey, shape, -radius, radius)`` followed by ``key, subkey =
+    split(key)``.  Sites given by init_to_value take their (unconstrained)
+    values and consume no key.  The next attempt continues from the key
+    the previous one ended with."""
+
+    def __init__(self, model, keys, radius=2.0, values=None):
+        self.layout = _site_layout(model)
+        self.dim = int(model.dim)
+        self.radius = np.float32(radius)
+        self.values = dict(values or {})
+        keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+        self.keys = _split_at(keys, 1)
+
+    def draw(self, c):
+        lo = -self.radius
+        key = self.keys[c]
+        key, sub = nrandom.split(key)
+        z = np.zeros(self.dim, np.float32)
+        for name, off, size, _ in self.layout:
+            if name in self.values:
+                v = self.values[name]
+                z[off:off + size] = v[c if v.shape[0] > 1 else 0]
+                continue
+            u = _uniform(sub, size).cpu().numpy()
+            # float32 f * (max - min) + min, clamped below by min
+            z[off:off + size] = np.maximum(lo, u * (self.radius - lo) + lo)
+            key, sub = nrandom.split(key)
+        self.keys[c] = key
+        return z
+
+
+def _split_at(keys, i):
+    """split(k)[i] for each key (one batched device call)."""
+    kd = torch.as_tensor(np.ascontiguousarray(keys).view(np.int32)).cuda().contiguous()
+    out = torch.empty((keys.shape[0], 2, 2), dtype=torch.int32, device="cuda")
+    _lib.check(_lib.load().nax_split_keys(_lib.ptr(kd), keys.shape[0], 2,
+                                          int(nrandom.PARTITIONABLE), _lib.ptr(out)),
+               "nax_split_keys")
+    return out[:, i].cpu().numpy().view(np.uint32).copy()
+
+
+def _initial_params(model, keys, values=None, radius=2.0, max_tries=100, eng=None):
+    """Initial points of all chains as find_valid_initial_params draws them,
+    redrawn (continuing each chain's key sequence) while U or dU/dz is not
+    finite there, at most 100 attempts (numpyro/infer/util.py:417-466)."""
+    draws = _InitKeys(model, keys, radius, values)
+    C_ = np.asarray(keys).reshape(-1, 2).shape[0]
+    z = torch.from_numpy(np.stack([draws.draw(c) for c in range(C_)]))
+    if eng is None:
+        return z
+    D = z.shape[1]
     ld = _lib.stride(eng.h)
     lib = _lib.load()
     for _ in range(max_tries):
@@ -611,23 +685,23 @@ def _find_valid_initial_params(model, z0, keys, eng, max_tries=100):
         if not bool(bad.any()):
             return z
         for c in torch.nonzero(bad).flatten().tolist():
-            keys[c] = nrandom.split(keys[c])[0]
-            z[c] = init_to_uniform(keys[c:c + 1], z.shape[1], self_split=True)[0]
+            z[c] = torch.from_numpy(draws.draw(c))
     raise RuntimeError("Cannot find valid initial parameters. Please check your model again.")
 
 
 def init_to_uniform(keys, dim, radius=2.0, self_split=False):
-    """Uniform(-radius, radius) initial points in unconstrained space
-    (``numpyro/infer/initialization.py:95-129``), one per chain key.  The
-    key is split as in ``HMC.init`` (``hmc.py:744-750``) and the second half
-    drives ``random.uniform``; the per-site key derivation of numpyro's
-    effect handlers is not reproduced."""
+    """Uniform(-radius, radius) draws of a flat latent of size ``dim``, one
+    per chain key: ``uniform(subkey, (dim,))`` with ``subkey =
+    split(k)[1]`` (``self_split=True`` first takes the second half of
+    ``split(key)`` as HMC.init does, hmc.py:744-750).  For models,
+    ``HMC.init`` draws site by site as find_valid_initial_params does."""
     keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
     rows = []
     for k in keys:
         kk = nrandom.split(k)[1] if self_split else k
-        u = _uniform(kk, dim)
-        rows.append(u * (2 * radius) - radius)
+        sub = nrandom.split(kk)[1]
+        u = _uniform(sub, dim)
+        rows.append(torch.clamp(u * (2 * radius) - radius, min=-radius))
     return torch.stack(rows)
 
 

@@ -88,9 +88,12 @@ class LogisticRegression(Model):
     (``examples/covtype.py:66-71``, ``test/infer/test_mcmc.py:104-168``).
 
     ``X`` [N, D] and ``y`` [N] are moved to the GPU as float32.  ``x_dtype``
-    picks the storage the gradient kernels read: ``"fp32"`` (plain) or
+    picks the storage the gradient kernels read: ``"fp32"`` (plain),
     ``"bf16"`` (MFMA; by default one bf16 copy of X with bf16 residuals in
-    X^T R, ``kernel="two-copy"`` adds an fp16 copy for fp16 residuals).  ``"fp8"`` is refused: both operands of an fp8 MFMA are
+    X^T R, ``kernel="two-copy"`` adds an fp16 copy for fp16 residuals) or
+    ``"fp8"`` (OCP e4m3fn bytes with a power-of-two scale per column, decoded
+    exactly in the VALU tile kernel, fp32 arithmetic; the quantised design
+    defines the model).  The fp8 MFMA is not used: both of its operands are
     8-bit, so the positions would need a three-term e4m3 split (as costly as
     the two bf16 products) and the residuals of X^T R would keep 3 mantissa
     bits."""
@@ -116,6 +119,8 @@ class LogisticRegression(Model):
         if x_dtype == "bf16" and kernel in ("tile", "chain"):
             raise ValueError("bf16 storage runs on the matrix cores: kernel 'auto', 'single' "
                              "or 'two-copy'")
+        if _canon(x_dtype) == "fp8" and kernel not in ("auto", "tile"):
+            raise ValueError("fp8 storage runs on the VALU tile kernel: kernel 'auto' or 'tile'")
         self.kernel = kernel
         self.site_sizes = (("coefs", self.dim, None),)
 
@@ -225,6 +230,8 @@ class StochasticVolatility(Model):
                 and math.isfinite(self.nu_rate)):
             raise ValueError("sigma_rate and nu_rate must be finite and positive")
         self.positive_transform = tf = _positive_kind(positive_transform)
+        # execution order of the sample statements (stochastic_volatility.py:58-66)
+        self.site_order = ("sigma", "zeta" if self.noncentred else "s", "nu")
         if self.noncentred:
             self.site_sizes = (("nu", 1, tf), ("sigma", 1, tf), ("zeta", self.T, None))
         else:

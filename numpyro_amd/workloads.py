@@ -83,12 +83,12 @@ def make_hier(G=1024, seed=4, nmin=50, nmax=5000, dmin=4, dmax=64):
     return Xs, ys
 
 
-def laplace(X, y, prior_scale=1.0, iters=12, chunk=1 << 18):
-    """Mode and diagonal covariance of the Laplace approximation of the
-    logistic posterior: Newton iterations on U(b) = -sum log Bernoulli(y |
+def laplace(X, y, prior_scale=1.0, iters=12, chunk=1 << 18, full_cov=False):
+    """Mode and covariance of the Laplace approximation of the logistic
+    posterior: Newton iterations on U(b) = -sum log Bernoulli(y |
     sigmoid(X b)) + |b|^2 / (2 s^2), the Hessian X^T diag(w) X + I / s^2
-    accumulated in float64 over row chunks.  Returns (mode, diag(H^-1)) as
-    float64 tensors on X's device."""
+    accumulated in float64 over row chunks.  Returns (mode, diag(H^-1)), or
+    (mode, H^-1) with ``full_cov``, as float64 tensors on X's device."""
     N, D = X.shape
     b = torch.zeros(D, dtype=torch.float64, device=X.device)
     eye = torch.eye(D, dtype=torch.float64, device=X.device) / prior_scale ** 2
@@ -104,16 +104,19 @@ def laplace(X, y, prior_scale=1.0, iters=12, chunk=1 << 18):
         b = b - step
         if float(step.abs().max()) < 1e-10:
             break
-    return b, torch.diagonal(torch.linalg.inv(H)).clone()
+    cov = torch.linalg.inv(H)
+    return b, (0.5 * (cov + cov.T) if full_cov else torch.diagonal(cov).clone())
 
 
-def laplace_init(X, y, C, seed=0):
+def laplace_init(X, y, C, seed=0, full_cov=False):
     """init_params (mode + sd * uniform(-2, 2) per coordinate and chain) and
-    the initial inverse_mass_matrix (Laplace variances), both float32."""
+    the initial inverse_mass_matrix (Laplace variances, or the Laplace
+    covariance with ``full_cov``), both float32."""
     import numpy as np
-    mode, var = laplace(X, y)
+    mode, cov = laplace(X, y, full_cov=full_cov)
+    var = torch.diagonal(cov) if full_cov else cov
     sd = var.sqrt()
     u = torch.as_tensor(np.random.default_rng(seed).uniform(-2.0, 2.0, (C, X.shape[1])),
                         device=X.device)
     z0 = (mode[None, :] + sd[None, :] * u).to(torch.float32).contiguous()
-    return z0, var.to(torch.float32).contiguous()
+    return z0, cov.to(torch.float32).contiguous()

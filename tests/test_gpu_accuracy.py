@@ -166,3 +166,42 @@ def test_accuracy_logistic_c5():
         if r["form"].startswith("bf16"):
             assert r["U_rel_err_vs_rounded_X"] < 1e-5, r
             assert r["grad_rel_err_vs_rounded_X"] < 5e-3, r
+
+
+def test_energy_differences_c5():
+    # Accept/reject and the tree weights use differences of energies between
+    # nearby positions (hmc_util.py:866-874), not U itself: at the full C5
+    # design (N = 2M, |U| ~ 1e6) the error of U(z') - U(z) against float64
+    # (on the same, bf16-rounded for the bf16 form, design) for steps of the
+    # size of a leapfrog move must stay far below one nat.
+    from numpyro_amd import workloads
+    from numpyro_amd.models import LogisticRegression, _device_value_and_grad
+    X, y, beta, cfg = workloads.make("c5")
+    C = 32
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(9)
+    _, var = workloads.laplace(X, y, iters=1)
+    sd = var.sqrt().float()
+    Z = (beta[None, :] + 0.01 * torch.randn((C, cfg["D"]), generator=gen, device="cuda"))
+    Z = Z.float().contiguous()
+    step = (0.5 * sd[None, :] * torch.randn((C, cfg["D"]), generator=gen, device="cuda"))
+    Z2 = (Z + step).float().contiguous()
+    recs = []
+    for form in ("fp32", "bf16"):
+        m = LogisticRegression(X, y, x_dtype=form, kernel="mfma" if form == "fp32" else "auto")
+        u1, _ = _device_value_and_grad(m, Z)
+        u2, _ = _device_value_and_grad(m, Z2)
+        Xr = X.to(torch.bfloat16).float() if form == "bf16" else X
+        a, _ = _logistic64(Xr, y, Z)
+        b, _ = _logistic64(Xr, y, Z2)
+        d = (u2 - u1).cpu().numpy()
+        d64 = b - a
+        err = np.abs(d - d64)
+        rec = {"config": "c5", "form": form, "chains": C, "max_abs_U": float(np.abs(a).max()),
+               "mean_abs_dU": float(np.abs(d64).mean()), "max_abs_dU_err": float(err.max()),
+               "max_abs_U_err": float(np.abs(u1.cpu().numpy() - a).max())}
+        recs.append(rec)
+        del m
+    _report(recs)
+    for r in recs:
+        assert r["max_abs_dU_err"] < 0.05, r

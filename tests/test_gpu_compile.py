@@ -129,11 +129,15 @@ def test_compiled_sv_and_hier_match_builtin():
         eta = npa.sample("eta", dist.Normal(torch.zeros(5), 1.0))
         npa.sample("y", dist.Bernoulli(logits=X @ (mu + tau * eta)), obs=y)
 
+    # the same given starting points (the built-in ragged model has no named
+    # sites, so its default initial draws differ from the traced model's)
+    z0 = torch.from_numpy(np.random.default_rng(1).uniform(-2, 2, (4, 7)).astype(np.float32))
     a = MCMC(NUTS(hier), num_warmup=60, num_samples=30, num_chains=4)
-    a.run(random.PRNGKey(6), X, y)
+    a.run(random.PRNGKey(6), X, y, init_params={"eta": z0[:, :5], "mu": z0[:, 5],
+                                                 "tau": z0[:, 6]})
     b = MCMC(NUTS(HierarchicalLogistic([X], [y], chains_per_group=4)), num_warmup=60,
              num_samples=30, num_chains=4)
-    b.run(random.PRNGKey(6))
+    b.run(random.PRNGKey(6), init_params=z0)
     sa, sb = a.get_samples(), b.get_samples()
     assert torch.equal(sa["mu"], sb["mu"]) and torch.equal(sa["tau"], sb["tau"])
     assert torch.equal(sa["eta"], sb["eta"][:, :5])
@@ -153,15 +157,18 @@ def test_compiled_intercept_matches_augmented_builtin():
         npa.sample("y", dist.Bernoulli(logits=X @ b + a), obs=y)
 
     # the flat latent is [a, b] (sorted site names, as ravel_pytree), so a
-    # flat inverse_mass_matrix applies its first entry to the intercept
+    # flat inverse_mass_matrix applies its first entry to the intercept; both
+    # start from the same given point (model-based init draws site by site in
+    # the model's order, the built-in model has one site)
     imm = torch.tensor([0.5, 1.0, 2.0, 3.0])
+    z0 = torch.tensor([[0.1, -0.2, 0.3, 0.0], [-0.3, 0.2, 0.1, -0.1]])
     m = MCMC(NUTS(model, inverse_mass_matrix=imm, adapt_mass_matrix=False), num_warmup=100,
              num_samples=100, num_chains=2)
-    m.run(random.PRNGKey(8), X, y)
+    m.run(random.PRNGKey(8), X, y, init_params={"a": z0[:, 0], "b": z0[:, 1:]})
     Xa = torch.cat([torch.ones(X.shape[0], 1), X], 1)
     r = MCMC(NUTS(LogisticRegression(Xa, y, prior_scale=1.5), inverse_mass_matrix=imm,
                   adapt_mass_matrix=False), num_warmup=100, num_samples=100, num_chains=2)
-    r.run(random.PRNGKey(8))
+    r.run(random.PRNGKey(8), init_params=z0)
     s, t = m.get_samples(), r.get_samples()["coefs"]
     assert set(s) == {"a", "b"}
     assert torch.equal(s["a"], t[:, 0]) and torch.equal(s["b"], t[:, 1:])

@@ -158,3 +158,32 @@ def test_hier_softplus_potential_grad():
         np.testing.assert_allclose(pe[c].item(), u, rtol=1e-5, atol=1e-4)
         np.testing.assert_allclose(g[c, :d + 2].cpu().numpy(), gr, rtol=1e-4,
                                    atol=1e-5 * (np.abs(gr).max() + 1))
+
+
+def test_hier_graph_replay_matches_direct_launches_and_reports_tuning():
+    # nax_run replays the persistent chunks (target step + launch) as one
+    # captured hipGraph; direct launches must give every draw bit for bit.
+    # The autotuned chains-per-workgroup split does not change any draw.
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import HierarchicalLogistic
+    shapes = [(300, 5), (90, 12), (1200, 3), (64, 7)]
+    Xs, ys = _groups(shapes, 23)
+    K = 4
+    C = len(shapes) * K
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(4), C)
+    outs = []
+    for graphs, shape in ((True, None), (False, None), (False, 1), (True, 2)):
+        eng = Engine(HierarchicalLogistic(Xs, ys, chains_per_group=K), C, "NUTS", 60)
+        eng.set_graphs(graphs)
+        if shape:
+            eng.set_launch_shape(shape)
+        eng.init(torch.zeros((C, eng.D)), keys)
+        out = eng.run(160, collect_fields=("z", "num_steps", "potential_energy"), size=160)
+        tun = eng.tuning()
+        assert tun.startswith("hier: ") and "chains per workgroup" in tun, tun
+        print(f"[hier graphs={graphs} shape={shape}] {tun}", flush=True)
+        outs.append({k: v.cpu().numpy() for k, v in out.items()})
+    for o in outs[1:]:
+        for k in outs[0]:
+            assert np.array_equal(outs[0][k], o[k]), k

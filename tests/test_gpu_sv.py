@@ -135,3 +135,31 @@ def test_sv_softplus_potential_grad(noncentred, T, C):
         np.testing.assert_allclose(pe[c].item(), u, rtol=2e-6, atol=1e-3)
         np.testing.assert_allclose(g[c].cpu().numpy(), gr, rtol=2e-4,
                                    atol=2e-5 * (np.abs(gr).max() + 1))
+
+
+def test_sv_graph_replay_matches_direct_launches_at_fixed_waves_per_chain():
+    # The persistent chunks replay as one captured hipGraph; with the waves
+    # per chain fixed, direct launches give every draw bit for bit.  The
+    # tuned shape is reported.
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import StochasticVolatility
+    T, C = 500, 6
+    r = _returns(T, 41)
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(2), C)
+    z0 = torch.from_numpy(_points(T, C, True, 9))
+    eng = Engine(StochasticVolatility(r, noncentred=True), C, "NUTS", 40)
+    eng.init(z0, keys)
+    eng.run(45)
+    assert "waves per chain" in eng.tuning(), eng.tuning()
+    for w in (4, 1):
+        outs = []
+        for graphs in (True, False):
+            e = Engine(StochasticVolatility(r, noncentred=True), C, "NUTS", 40)
+            e.set_graphs(graphs)
+            e.set_launch_shape(w)
+            e.init(z0, keys)
+            out = e.run(120, collect_fields=("z", "num_steps", "diverging"), size=120)
+            outs.append({k: v.cpu().numpy() for k, v in out.items()})
+        for k in outs[0]:
+            assert np.array_equal(outs[0][k], outs[1][k]), (w, k)
