@@ -292,6 +292,23 @@ int nax_set_adapt_state(nax_sampler* h, const nax_adapt_view* v);
  * then x_q = e4m3(code) * s_d, exact in fp32. */
 int nax_fp8_quantize(const float* d_x, int64_t n, int32_t d, uint8_t* d_codes, float* d_scales);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the potential and gradient of the logistic model
+ * (coefs ~ Normal(0, prior_scale), obs ~ Bernoulli(logits = X @ coefs)) at
+ * d_z [count, ld] fp32 -> d_pe [count] fp64, d_grad [count, ld] fp32, with
+ * the kernels of nax_potential_grad on a logistic model in the fp32 tile
+ * form (nax_model_logistic_ex kernel 1), without a sampler handle.  d_x
+ * [n, d] fp32 row-major, d_y [n] fp32; d <= 320.  Allocates nothing: the
+ * row-split partials live in a 16 MiB device array of the library (one per
+ * GPU, held until the process ends); count * (8 + 4 d) bytes must fit it
+ * (else NAX_EINVAL).  Asynchronous on `stream` (a hipStream_t; NULL = the
+ * null stream).  Calls that overlap on one GPU share that array and race. */
+int nax_logistic_potential_grad_noalloc(const float* d_x, const float* d_y, int64_t n, int32_t d,
+                                        double prior_scale, const float* d_z, int32_t ld,
+                                        int32_t count, double* d_pe, float* d_grad,
+                                        void* stream);
+
 /* Potential and gradient of the built-in model at d_z [count, ld]:
  * potential_energy + value_and_grad (util.py:333-358, hmc_util.py:242-252). */
 int nax_potential_grad(nax_sampler* h, const float* d_z, int32_t ld, int32_t count,

@@ -99,6 +99,10 @@ SIGNATURES = {
     "nax_set_adapt_state": (C.c_int, [C.c_void_p, C.POINTER(nax_adapt_view)]),
     "nax_potential_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                      C.c_void_p, C.c_void_p]),
+    "nax_logistic_potential_grad_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64,
+                                                      C.c_int32, C.c_double, C.c_void_p,
+                                                      C.c_int32, C.c_int32, C.c_void_p,
+                                                      C.c_void_p, C.c_void_p]),
     "nax_leapfrog": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_double, C.c_int32, C.c_void_p]),
     "nax_threefry2x32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

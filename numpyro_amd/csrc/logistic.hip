@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "nax_device.h"
@@ -333,6 +334,54 @@ struct LogisticPotential : Potential {
   }
 };
 
+// ---- the allocation-free entry point ---------------------------------------------
+// The row-split partials of logit_partial_f32 / logit_reduce live in this
+// file-scope device array instead of a hipMalloc'd workspace: fp64 energy
+// partials [RB][C] first, then fp32 gradient partials [RB][C][D].  Each GPU
+// holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel: it passes the device address that
+// hipGetSymbolAddress returned for the current device.
+constexpr size_t kScratchBytes = size_t(16) << 20;  // 16 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current).
+static hipError_t scratch_address(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+// Row splits for `count` chains: the VALU tile kernel's rule (as the
+// handle-based fp32 tile potential uses it), reduced until the partials fit
+// the scratch array; 0 when even one split does not fit.
+static static int noalloc_row_splits(int64_t n, int d, int count) {
+  constexpr int BN = 32, BC = 8;
+  const int ctiles = (count + BC - 1) / BC;
+  const int64_t ntiles = (n + BN - 1) / BN;
+  int64_t rb = std::min<int64_t>(ntiles, std::max<int64_t>(1, 2048 / ctiles));
+  const size_t per = (size_t)count * (sizeof(double) + sizeof(float) * (size_t)d);
+  const size_t fit = per ? (kScratchBytes - 256) / per : 0;
+  if (fit < 1) return 0;
+  return (int)std::min<int64_t>(rb, (int64_t)fit);
+}
+
 // The e4m3fn value of every code (OCP FP8: no infinities, S.1111.111 NaN).
 static void fill_e4m3_table(float* t) {
   for (int c = 0; c < 256; ++c) {
@@ -487,6 +536,74 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
 }
 
 }  // namespace nax
+
+extern "C" int nax_logistic_potential_grad_noalloc(const float* d_x, const float* d_y, int64_t n,
+                                                   int32_t d, double prior_scale,
+                                                   const float* d_z, int32_t ld, int32_t count,
+                                                   double* d_pe, float* d_grad, void* stream) {
+  using namespace nax;
+  if (!d_x || !d_y || !d_z || !d_pe || !d_grad || n < 1 || d < 1 || count < 1 || ld < d) {
+    set_error("nax_logistic_potential_grad_noalloc: need x, y, z, pe, grad, n >= 1, d >= 1, "
+              "count >= 1, ld >= d");
+    return NAX_EINVAL;
+  }
+  if (d > 320) {
+    set_error("nax_logistic_potential_grad_noalloc: d <= 320 (the fp32 tile kernel)");
+    return NAX_EINVAL;
+  }
+  if (!(prior_scale > 0.0) || !std::isfinite(prior_scale)) {
+    set_error("nax_logistic_potential_grad_noalloc: prior_scale must be finite and > 0");
+    return NAX_EINVAL;
+  }
+  const int RB = noalloc_row_splits(n, d, count);
+  if (RB < 1) {
+    set_error("nax_logistic_potential_grad_noalloc: count * (8 + 4 d) bytes of partials exceed "
+              "the 16 MiB scratch array (count * d at most about 4.19e6)");
+    return NAX_EINVAL;
+  }
+  unsigned char* base = nullptr;
+  hipError_t e = scratch_address(&base);
+  if (e != hipSuccess) {
+    set_error(std::string("nax_logistic_potential_grad_noalloc: hipGetSymbolAddress: ") +
+              hipGetErrorString(e));
+    return NAX_EHIP;
+  }
+  double* wsu = reinterpret_cast<double*>(base);
+  const size_t ubytes = ((size_t)RB * count * sizeof(double) + 255) / 256 * 256;
+  float* wsg = reinterpret_cast<float*>(base + ubytes);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  constexpr int BN = 32, BC = 8;
+  const int ctiles = (count + BC - 1) / BC;
+  const int dch = (d + 63) / 64;
+  const int DP = dch * 64 + 1;
+  const size_t shm =
+      sizeof(float) * ((size_t)BC * DP + (size_t)BN * DP + BN * BC) + 256 * sizeof(double);
+  dim3 grid(RB, ctiles);
+#define NAX_LAUNCH(DC)                                                                      \
+  hipLaunchKernelGGL((logit_partial_f32<BN, BC, DC, XF32>), grid, dim3(256), shm, s, XF32{d_x},  \
+                     d_y, n, d, d_z, ld, count, RB, wsg, wsu)
+  switch (dch) {
+    case 1: NAX_LAUNCH(1); break;
+    case 2: NAX_LAUNCH(2); break;
+    case 3: NAX_LAUNCH(3); break;
+    case 4: NAX_LAUNCH(4); break;
+    case 5: NAX_LAUNCH(5); break;
+  }
+#undef NAX_LAUNCH
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    const double inv_s2 = 1.0 / (prior_scale * prior_scale);
+    const double log_norm = std::log(std::sqrt(2.0 * M_PI) * prior_scale);
+    hipLaunchKernelGGL(logit_reduce, dim3(count), dim3(256), 0, s, wsg, wsu, RB, count, d, d_z,
+                       ld, inv_s2, log_norm, 0.0, d_pe, d_grad, ld, nullptr, nullptr, 0, 0);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("nax_logistic_potential_grad_noalloc: ") + hipGetErrorString(e));
+    return NAX_EHIP;
+  }
+  return NAX_OK;
+}
 
 extern "C" int nax_fp8_quantize(const float* d_x, int64_t n, int32_t d, uint8_t* d_codes,
                                 float* d_scales) {
