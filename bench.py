@@ -41,13 +41,17 @@ def log(*a):
 # 1.01 and a long tail of deep trees), whatever --warmup asks for.
 MIN_ADAPT = 150
 
+# (timed, warmup) iterations per workload when not given
+DEFAULT_ITERS = {"c5": (120, 150), "c2": (300, 300), "c1": (500, 500), "c3": (600, 600),
+                 "c4": (500, 500), "ecs": (100, 100)}
+
 
 def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True, imm=None,
-             heuristic=False, dense=False):
+             heuristic=False, dense=False, adapt_mass=True):
     import torch
     from numpyro_amd.infer.hmc import Engine
     eng = Engine(model, C, "NUTS", num_warmup, find_heuristic_step_size=heuristic,
-                 dense_mass=dense)
+                 dense_mass=dense, adapt_mass_matrix=adapt_mass)
     eng.init(z0, keys, imm)
     t_start = time.perf_counter()
     eng.set_progress(lambda ticks, pending: log(
@@ -146,13 +150,18 @@ def cpu_baseline(X, y, z0, budget_s, threads, chains=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=120)
-    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed iterations (default per config: c5 120, c2 300, c1 500, "
+                         "c3 600, c4 500, ecs 100)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed warmup iterations (default per config: c5 150, c2 300, "
+                         "c1 500, c3 600, c4 500, ecs 100)")
     ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4", "ecs"])
     ap.add_argument("--subsample", type=int, default=1000, help="ecs: subsample size m")
     ap.add_argument("--num-blocks", type=int, default=100, help="ecs: Gibbs blocks")
     ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
-    ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--per-model", default=None, help="c4: write per-model diagnostics (JSON)")
+    ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32", "fp8"])
     ap.add_argument("--bf16-kernel", default="single", choices=["single", "two-copy"],
                     help="bf16 storage: one X copy (bf16 residuals, the library default) or "
                          "two (fp16 residuals)")
@@ -165,7 +174,13 @@ def main():
     ap.add_argument("--dense-mass", action="store_true",
                     help="logistic workloads: dense_mass=True, initial inverse_mass_matrix "
                          "the Laplace covariance")
+    ap.add_argument("--mass", default="laplace", choices=["laplace", "adapt"],
+                    help="logistic workloads: keep the Laplace inverse mass matrix fixed "
+                         "(adapt_mass_matrix=False), or adapt it by Welford windows from it")
     args = ap.parse_args()
+    steps, warm = DEFAULT_ITERS[args.config]
+    args.steps = steps if args.steps is None else args.steps
+    args.warmup = warm if args.warmup is None else args.warmup
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); chains "
               "would shard across GPUs with no communication", file=sys.stderr)
@@ -190,7 +205,8 @@ def main():
     log(f"workload {args.config}: N={N} D={D} chains={C} x_dtype={args.x_dtype}")
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
     t0 = time.perf_counter()
-    z0, imm = workloads.laplace_init(X, y, C, full_cov=args.dense_mass)
+    z0, imm, mode, lsd = workloads.laplace_init(X, y, C, full_cov=args.dense_mass,
+                                                 return_mode=True)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     imd = torch.diagonal(imm) if imm.ndim == 2 else imm
@@ -201,8 +217,10 @@ def main():
         X, y, x_dtype=args.x_dtype,
         kernel=args.bf16_kernel if args.x_dtype == "bf16" else "auto")
     nw = max(args.warmup, MIN_ADAPT)
+    adapt_mass = args.mass == "adapt"
     eng, out, wall, warm_s, kms, kn = run_form(model, C, nw, args.steps, z0, keys,
-                                               imm=imm, heuristic=True, dense=args.dense_mass)
+                                               imm=imm, heuristic=True, dense=args.dense_mass,
+                                               adapt_mass=adapt_mass)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
     div = int(out["diverging"].sum())
@@ -210,6 +228,18 @@ def main():
     mess, rhat = min_ess(out["z"])
     gps = ge / wall
     valid = not degenerate(div, acc, C * args.steps)
+    ss = eng.get_state().adapt_state.step_size.double().cpu().numpy()
+    leaves = ns.sum(1).astype(np.float64)
+    spread = {"step_size_q": [float(np.quantile(ss, q)) for q in (0.0, 0.01, 0.5, 0.99, 1.0)],
+              "chain_leaves_q": [float(np.quantile(leaves, q)) for q in (0.0, 0.01, 0.5, 0.99, 1.0)],
+              "slowest_over_mean_chain": float(leaves.max() / max(leaves.mean(), 1e-9))}
+    log(f"per-chain spread: {spread}")
+    # posterior mean of the chains against the Laplace mode of the fp32 design,
+    # in Laplace standard deviations: how far this storage form moves the
+    # posterior (X rounded to bf16 / e4m3 defines a slightly different model)
+    shift = ((out["z"].double().mean((0, 1)) - mode) / lsd).abs()
+    spread["mean_shift_vs_fp32_laplace_mode_sd"] = {"max": float(shift.max()),
+                                                    "rms": float(shift.pow(2).mean().sqrt())}
     log(f"{args.x_dtype}: warmup {warm_s:.1f}s, timed {wall:.2f}s, grad evals {ge}, "
         f"{gps:.4g}/s, min ESS {mess:.1f}, max R-hat {rhat:.3f}, mean tree {ns.mean():.1f}, "
         f"accept {acc:.3f}, divergences {div}, step size "
@@ -220,14 +250,16 @@ def main():
     k_avg_ms = kms / max(kn, 1)
     flops = model.algorithmic_flops(1) * ge / max(kn, 1)
     achieved = model.algorithmic_flops(1) * ge / (kms * 1e-3) / 1e12
-    peak = PEAKS[args.x_dtype]
+    # the fp8 storage form decodes X to fp32 and runs on the VALU
+    peak = PEAKS["fp32" if args.x_dtype == "fp8" else args.x_dtype]
     post_state = eng.get_state()
 
     fp32 = None
     if args.fp32_steps > 0 and args.x_dtype != "fp32" and valid:
         m32 = LogisticRegression(X, y, x_dtype="fp32", kernel="mfma")
         e32, o32, w32, _, k32, n32 = run_form(m32, C, nw, args.fp32_steps, z0, keys,
-                                              state=post_state, dense=args.dense_mass)
+                                              state=post_state, dense=args.dense_mass,
+                                              adapt_mass=adapt_mass)
         me32, rh32 = min_ess(o32["z"]) if args.fp32_steps >= 4 else (float("nan"), float("nan"))
         # the bf16 chains' ESS over the same number of draws, for comparison
         mes_b, _ = min_ess(out["z"][:, :args.fp32_steps])
@@ -273,7 +305,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.x_dtype,
+        "dtype": "fp32" if args.x_dtype == "fp8" else args.x_dtype,
         "data": "synthetic",
         "config": {"workload": f"logistic NUTS {args.config}: N={N}, D={D}, chains={C}, "
                                f"X {args.x_dtype}, max_tree_depth 10, target_accept 0.8, "
@@ -283,6 +315,7 @@ def main():
                    "init_params": "Laplace mode + Laplace sd * uniform(-2, 2) per chain",
                    "inverse_mass_matrix": ("Laplace covariance (dense_mass=True)"
                                            if args.dense_mass else "Laplace variances (diagonal)"),
+                   "adapt_mass_matrix": adapt_mass,
                    "find_heuristic_step_size": True},
         "valid": valid,
         "setup_s": setup_s,
@@ -292,6 +325,7 @@ def main():
         "min_ess_per_iteration": mess / args.steps,
         "max_rhat": rhat,
         "mean_tree_size": float(ns.mean()),
+        "chains": spread,
         "mean_accept_prob": acc,
         "divergences": div,
         "warmup_s": warm_s,
@@ -341,6 +375,7 @@ def bench_sv(args):
                        "global_batch": C, "seq_len": T, "parallelism": "chains"},
             "min_ess_per_s": mess / wall, "min_ess": mess, "max_rhat": rhat,
             "mean_tree_size": float(out["num_steps"].float().mean()),
+            "straggler_ratio_50": straggler_ratio(out["num_steps"].cpu().numpy()),
             "mean_accept_prob": float(out["accept_prob"].float().mean()),
             "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
             "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
@@ -380,6 +415,7 @@ def bench_hier(args):
                            "min ESS / max split R-hat over the 1024 models",
             "groups_rhat_above_1.01": per["n_bad"], "worst_groups": per["worst"],
             "mean_tree_size": float(out["num_steps"].float().mean()),
+            "straggler_ratio_50": straggler_ratio(out["num_steps"].cpu().numpy()),
             "mean_accept_prob": float(out["accept_prob"].float().mean()),
             "divergences": int(out["diverging"].sum()), "warmup_s": warm_s,
             "persistent_launch_avg_ms": kms / max(kn, 1), "launches": kn,
@@ -388,6 +424,31 @@ def bench_hier(args):
             "hbm_GBps_at_grad_rate": model.algorithmic_bytes(C) * ge / C / wall / 1e9}
     log(json.dumps(line))
     print(json.dumps(line), flush=True)
+    if args.per_model:
+        # per model: split R-hat / ESS over its own chains, and per chain the
+        # mean accept probability and the mean of the log-tau coordinate
+        acc = out["accept_prob"].float().mean(1).reshape(G, 4).cpu().numpy()
+        zc = out["z"].float().mean(1).cpu().numpy()
+        lt = np.array([zc[c, model.dims[c // 4] + 1] for c in range(C)]).reshape(G, 4)
+        rows = [{"group": g, "N": int(Xs[g].shape[0]), "D": int(model.dims[g]),
+                 "max_rhat": float(per["rhat_g"][g]), "min_ess": float(per["ess_g"][g]),
+                 "accept": acc[g].round(4).tolist(), "logtau_mean": lt[g].round(4).tolist()}
+                for g in range(G)]
+        with open(args.per_model, "w") as f:
+            json.dump(rows, f)
+
+
+def straggler_ratio(num_steps, chunk=50):
+    """Mean over chunks of `chunk` iterations of (leaves of the slowest
+    chain in the chunk) / (mean leaves per chain in the chunk): the factor
+    by which a launch that waits for its slowest chain exceeds the mean
+    chain's work."""
+    ns = np.asarray(num_steps, dtype=np.float64)
+    r = []
+    for i in range(0, ns.shape[1] - chunk + 1, chunk):
+        tot = ns[:, i:i + chunk].sum(1)
+        r.append(tot.max() / max(tot.mean(), 1e-9))
+    return float(np.mean(r)) if r else float("nan")
 
 
 def group_diagnostics(z, dims, K):
@@ -408,7 +469,7 @@ def group_diagnostics(z, dims, K):
     rhat_g = np.where(mask, rhat, -np.inf).max(1)
     order = np.argsort(-rhat_g)[:5]
     return {"min_ess": float(ess_g.min()), "max_rhat": float(rhat_g.max()),
-            "n_bad": int((rhat_g > 1.01).sum()),
+            "n_bad": int((rhat_g > 1.01).sum()), "rhat_g": rhat_g, "ess_g": ess_g,
             "worst": [{"group": int(g), "D": int(dims[g]), "max_rhat": float(rhat_g[g]),
                        "min_ess": float(ess_g[g])} for g in order]}
 
@@ -460,6 +521,7 @@ def bench_ecs(args):
                        "global_batch": C, "seq_len": m, "parallelism": "chains"},
             "min_ess_per_s": mess / wall, "min_ess": mess, "max_rhat": rhat,
             "mean_tree_size": float(out["num_steps"].float().mean()),
+            "straggler_ratio_50": straggler_ratio(out["num_steps"].cpu().numpy()),
             "mean_accept_prob": float(out["accept_prob"].float().mean()),
             "divergences": int(out["diverging"].sum()),
             "gibbs_accept_prob_mean": float(eng.gibbs_accept.mean()),
@@ -480,6 +542,8 @@ def model_kernel_name(x_dtype, N, D, C, single=False):
     dp = (D + 31) // 32 * 32
     if x_dtype == "bf16":
         return f"logit_bf16_pipe<{dp}, {'true' if single else 'false'}>"
+    if x_dtype == "fp8":
+        return "logit_partial_f32<XFP8> (VALU tile, e4m3 decoded to fp32)"
     if C <= 16 and D <= 64 and N * D <= (1 << 20):
         return (f"persistent_kernel<LogisticChainModel<{32 if D <= 32 else 64}>> "
                 "(one wavefront per chain; a launch runs 50 NUTS iterations)")

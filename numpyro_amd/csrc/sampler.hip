@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -676,14 +677,18 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
   Collect col = to_collect(c);
   NAX_HIP(hipMemsetAsync(h->gevals, 0, sizeof(int64_t), h->stream));
   if (h->pot->persistent()) {
-    // Whole loop on the device, in launches of at most kPersistIters
-    // iterations so that progress can be reported between them.  The
-    // iteration target lives in device memory (gevals[1]): each chunk is a
-    // target step kernel plus the persistent launch, enqueued directly or
-    // replayed as one hipGraph captured on the first chunk (the model's
-    // autotuning runs before, uncaptured).  Kernel timing brackets the main
-    // kernel with events, so it runs uncaptured.
-    constexpr int64_t kPersistIters = 50;
+    // Whole loop on the device, in chunks of iterations with progress
+    // reported between them.  Every chunk ends when its slowest chain does,
+    // so chunks are long: the first is kPersistIters iterations, later ones
+    // are sized from its duration to about kChunkMs of work (at most
+    // kMaxChunk iterations).  The iteration target lives in device memory
+    // (gevals[1]): a chunk is a target step kernel plus the persistent
+    // launch, enqueued directly or replayed as a hipGraph captured for the
+    // current chunk length (the model's autotuning runs before, uncaptured).
+    // Kernel timing brackets the main kernel with events, so it runs
+    // uncaptured.
+    constexpr int64_t kPersistIters = 50, kMaxChunk = 1000;
+    constexpr double kChunkMs = 1000.0;
     std::vector<ChainState> probe(1);
     NAX_HIP(hipMemcpyAsync(probe.data(), h->st, sizeof(ChainState), hipMemcpyDeviceToHost, h->stream));
     NAX_HIP(hipStreamSynchronize(h->stream));
@@ -693,9 +698,10 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
     int64_t t = probe[0].i;
     hipLaunchKernelGGL(set_target_kernel, dim3(1), dim3(1), 0, h->stream, d_target, t);
     NAX_HIP(hipGetLastError());
+    int64_t step = kPersistIters;
     auto chunk = [&]() -> hipError_t {
       hipLaunchKernelGGL(step_target_kernel, dim3(1), dim3(1), 0, h->stream, d_target, target,
-                         kPersistIters);
+                         step);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       return h->pot->run_persistent(h->stream, h->cfg, col, d_target, h->st, h->vec, h->wel,
@@ -709,8 +715,9 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
     } guard{&exec};
     int64_t launches = 0;
     while (t < target) {
-      t = std::min<int64_t>(target, t + kPersistIters);
+      t = std::min<int64_t>(target, t + step);
       const int k = (int)(launches % kBatch);
+      const auto wall0 = std::chrono::steady_clock::now();
       if (use_graph) {
         if (!exec) {
           NAX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -736,6 +743,20 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
       if (!use_graph) {
         const int rc = collect_timing(h, 1, k);
         if (rc) return rc;
+      }
+      if (launches == 0 && t < target) {
+        const double ms = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - wall0).count();
+        const int64_t next = std::max<int64_t>(
+            kPersistIters, std::min<int64_t>(kMaxChunk, (int64_t)(step * kChunkMs /
+                                                                   std::max(ms, 1e-3))));
+        if (next != step) {
+          step = next;
+          if (exec) {  // captured with the old chunk length
+            hipGraphExecDestroy(exec);
+            exec = nullptr;
+          }
+        }
       }
       ++launches;
       if (h->progress_cb) h->progress_cb(h->progress_user, launches, h->cfg.C);

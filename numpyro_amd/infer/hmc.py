@@ -268,9 +268,17 @@ class Engine:
             h["key"].ctypes.data)
         torch.cuda.synchronize()
         _lib.check(self.lib.nax_get_state(self.h, C.byref(v)), "nax_get_state")
-        # mass_matrix_sqrt_inv: 1 / sqrt(M) (diagonal) or tril_inv = S^-1 (dense)
-        msi = (torch.linalg.inv(t["msqrt"].double()).float() if self.dense
-               else 1.0 / t["msqrt"])
+        # mass_matrix_sqrt_inv: 1 / sqrt(M) (diagonal) or tril_inv = S^-1 (dense;
+        # S is lower triangular: triangular solves, 64 chains at a time)
+        if self.dense:
+            msi = torch.empty_like(t["msqrt"])
+            eye = torch.eye(D, dtype=torch.float64, device="cuda")
+            for c0 in range(0, Cn, 64):
+                blk = t["msqrt"][c0:c0 + 64].double()
+                msi[c0:c0 + 64] = torch.linalg.solve_triangular(
+                    blk, eye.expand_as(blk), upper=False).float()
+        else:
+            msi = 1.0 / t["msqrt"]
         ss_state, mm_state, window_idx, akey = self._get_adapt()
         adapt = HMCAdaptState(torch.from_numpy(h["ss"]), t["imm"], t["msqrt"],
                               msi, ss_state, mm_state, window_idx, akey)

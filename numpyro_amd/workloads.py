@@ -108,10 +108,11 @@ def laplace(X, y, prior_scale=1.0, iters=12, chunk=1 << 18, full_cov=False):
     return b, (0.5 * (cov + cov.T) if full_cov else torch.diagonal(cov).clone())
 
 
-def laplace_init(X, y, C, seed=0, full_cov=False):
+def laplace_init(X, y, C, seed=0, full_cov=False, return_mode=False):
     """init_params (mode + sd * uniform(-2, 2) per coordinate and chain) and
     the initial inverse_mass_matrix (Laplace variances, or the Laplace
-    covariance with ``full_cov``), both float32."""
+    covariance with ``full_cov``), both float32; with ``return_mode`` also
+    the float64 mode and standard deviations."""
     import numpy as np
     mode, cov = laplace(X, y, full_cov=full_cov)
     var = torch.diagonal(cov) if full_cov else cov
@@ -119,4 +120,6 @@ def laplace_init(X, y, C, seed=0, full_cov=False):
     u = torch.as_tensor(np.random.default_rng(seed).uniform(-2.0, 2.0, (C, X.shape[1])),
                         device=X.device)
     z0 = (mode[None, :] + sd[None, :] * u).to(torch.float32).contiguous()
+    if return_mode:
+        return z0, cov.to(torch.float32).contiguous(), mode, sd
     return z0, cov.to(torch.float32).contiguous()

@@ -172,20 +172,25 @@ def test_energy_differences_c5():
     # Accept/reject and the tree weights use differences of energies between
     # nearby positions (hmc_util.py:866-874), not U itself: at the full C5
     # design (N = 2M, |U| ~ 1e6) the error of U(z') - U(z) against float64
-    # (on the same, bf16-rounded for the bf16 form, design) for steps of the
-    # size of a leapfrog move must stay far below one nat.
+    # (on the same design, bf16-rounded for the bf16 form) for steps of the
+    # size of a leapfrog move must stay far below one nat.  Positions are
+    # drawn where the chains are: the Laplace mode plus one Laplace standard
+    # deviation of Gaussian noise per coordinate; steps are half a standard
+    # deviation.  (Far outside the posterior the gradient is ~100x larger
+    # and so is the energy error of the bf16 form's ~2^-17 position
+    # rounding: U errors of ~0.4 nats at 0.01 from beta*, tens of sds.)
     from numpyro_amd import workloads
     from numpyro_amd.models import LogisticRegression, _device_value_and_grad
     X, y, beta, cfg = workloads.make("c5")
     C = 32
     gen = torch.Generator(device="cuda")
     gen.manual_seed(9)
-    _, var = workloads.laplace(X, y, iters=1)
-    sd = var.sqrt().float()
-    Z = (beta[None, :] + 0.01 * torch.randn((C, cfg["D"]), generator=gen, device="cuda"))
-    Z = Z.float().contiguous()
-    step = (0.5 * sd[None, :] * torch.randn((C, cfg["D"]), generator=gen, device="cuda"))
-    Z2 = (Z + step).float().contiguous()
+    mode, var = workloads.laplace(X, y)
+    sd = var.sqrt()
+    noise = torch.randn((C, cfg["D"]), generator=gen, device="cuda", dtype=torch.float64)
+    Z = (mode[None, :] + sd[None, :] * noise).float().contiguous()
+    step = torch.randn((C, cfg["D"]), generator=gen, device="cuda", dtype=torch.float64)
+    Z2 = (Z.double() + 0.5 * sd[None, :] * step).float().contiguous()
     recs = []
     for form in ("fp32", "bf16"):
         m = LogisticRegression(X, y, x_dtype=form, kernel="mfma" if form == "fp32" else "auto")
