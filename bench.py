@@ -8,13 +8,14 @@ each chain's initial point (the mode plus a uniform(-2, 2) jitter in units
 of the Laplace standard deviation, passed as init_params) and the initial
 diagonal inverse_mass_matrix (the Laplace variances); the sampler runs with
 find_heuristic_step_size=True.  --warmup W untimed iterations follow (dual
-averaging + diagonal Welford on the window schedule of W), then K timed
-sampling iterations.  value = gradient evaluations per second over the timed
+averaging + diagonal Welford on the window schedule of W; --mass laplace
+keeps the Laplace matrix fixed instead), then K timed sampling iterations.  value = gradient evaluations per second over the timed
 region (useful leapfrog steps summed over chains); min-ESS/s is computed
 from the K timed draws of every chain.  The BF16-MFMA form is the measured
 line; the plain FP32 form (same adapted state, same kernels otherwise) is
-timed beside it, and the CPU baseline (oracle/libcpu_nuts.so, a C/OpenMP
-restatement of the same NUTS path) on a bounded sample.  A run whose timed
+timed beside it, and the CPU baseline (the batched potential + gradient of
+oracle/libcpu_nuts.so, the C/OpenMP restatement of the same path, on the
+host's cores) on a bounded sample.  A run whose timed
 transitions mostly diverge (or accept almost nothing) prints its line with
 "valid": false and exits with status 3.
 """
@@ -42,16 +43,18 @@ def log(*a):
 MIN_ADAPT = 150
 
 # (timed, warmup) iterations per workload when not given
-DEFAULT_ITERS = {"c5": (120, 150), "c2": (300, 300), "c1": (500, 500), "c3": (600, 600),
+DEFAULT_ITERS = {"c5": (120, 150), "c2": (1000, 1000), "c1": (500, 500), "c3": (600, 600),
                  "c4": (500, 500), "ecs": (100, 100)}
 
 
 def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True, imm=None,
-             heuristic=False, dense=False, adapt_mass=True):
+             heuristic=False, dense=False, adapt_mass=True, launch_shape=None):
     import torch
     from numpyro_amd.infer.hmc import Engine
     eng = Engine(model, C, "NUTS", num_warmup, find_heuristic_step_size=heuristic,
                  dense_mass=dense, adapt_mass_matrix=adapt_mass)
+    if launch_shape:
+        eng.set_launch_shape(launch_shape)
     eng.init(z0, keys, imm)
     t_start = time.perf_counter()
     eng.set_progress(lambda ticks, pending: log(
@@ -151,17 +154,22 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed iterations (default per config: c5 120, c2 300, c1 500, "
+                    help="timed iterations (default per config: c5 120, c2 1000, c1 500, "
                          "c3 600, c4 500, ecs 100)")
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed warmup iterations (default per config: c5 150, c2 300, "
+                    help="untimed warmup iterations (default per config: c5 150, c2 1000, "
                          "c1 500, c3 600, c4 500, ecs 100)")
     ap.add_argument("--config", default="c5", choices=["c5", "c2", "c1", "c3", "c4", "ecs"])
     ap.add_argument("--subsample", type=int, default=1000, help="ecs: subsample size m")
     ap.add_argument("--num-blocks", type=int, default=100, help="ecs: Gibbs blocks")
     ap.add_argument("--noncentred", action="store_true", help="c3: non-centred SV form")
     ap.add_argument("--per-model", default=None, help="c4: write per-model diagnostics (JSON)")
-    ap.add_argument("--x-dtype", default="bf16", choices=["bf16", "fp32", "fp8"])
+    ap.add_argument("--launch-shape", type=int, default=None,
+                    help="persistent models (c1, c3, c4): fix the launch shape instead of "
+                         "autotuning it (waves per chain; c4: chains per workgroup + 32 x "
+                         "(waves per chain - 1))")
+    ap.add_argument("--x-dtype", default=None, choices=["bf16", "fp32", "fp8"],
+                    help="storage form of X (default: bf16 for c5, fp32 for c1 and c2)")
     ap.add_argument("--bf16-kernel", default="single", choices=["single", "two-copy"],
                     help="bf16 storage: one X copy (bf16 residuals, the library default) or "
                          "two (fp16 residuals)")
@@ -174,9 +182,10 @@ def main():
     ap.add_argument("--dense-mass", action="store_true",
                     help="logistic workloads: dense_mass=True, initial inverse_mass_matrix "
                          "the Laplace covariance")
-    ap.add_argument("--mass", default="laplace", choices=["laplace", "adapt"],
-                    help="logistic workloads: keep the Laplace inverse mass matrix fixed "
-                         "(adapt_mass_matrix=False), or adapt it by Welford windows from it")
+    ap.add_argument("--mass", default="adapt", choices=["adapt", "laplace"],
+                    help="logistic workloads: adapt the inverse mass matrix by Welford "
+                         "windows starting from the Laplace variances (default), or keep "
+                         "the Laplace matrix fixed (adapt_mass_matrix=False)")
     args = ap.parse_args()
     steps, warm = DEFAULT_ITERS[args.config]
     args.steps = steps if args.steps is None else args.steps
@@ -198,8 +207,10 @@ def main():
         return bench_hier(args)
     if args.config == "ecs":
         return bench_ecs(args)
-    if args.config in ("c1", "c2") and args.x_dtype == "bf16" and args.config == "c1":
-        args.x_dtype = "fp32"  # 4 chains: the VALU tile kernel
+    if args.x_dtype is None:
+        # the configurations as stated: C5 in bf16 (the plain-FP32 form timed
+        # beside it), C1 and C2 plain fp32
+        args.x_dtype = "bf16" if args.config == "c5" else "fp32"
     X, y, beta, cfg = workloads.make(args.config, N=args.rows, chains=args.chains)
     N, D, C = cfg["N"], cfg["D"], cfg["chains"]
     log(f"workload {args.config}: N={N} D={D} chains={C} x_dtype={args.x_dtype}")
@@ -220,7 +231,8 @@ def main():
     adapt_mass = args.mass == "adapt"
     eng, out, wall, warm_s, kms, kn = run_form(model, C, nw, args.steps, z0, keys,
                                                imm=imm, heuristic=True, dense=args.dense_mass,
-                                               adapt_mass=adapt_mass)
+                                               adapt_mass=adapt_mass,
+                                               launch_shape=args.launch_shape)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
     div = int(out["diverging"].sum())
@@ -362,7 +374,8 @@ def bench_sv(args):
     z0 = torch.zeros((C, T + 2), dtype=torch.float32, device="cuda")
     z0[:, 0] = float(np.log(5.0))
     z0[:, 1 if args.noncentred else T + 1] = float(np.log(0.02))
-    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys,
+                                               launch_shape=args.launch_shape)
     mess, rhat = min_ess(out["z"])
     ge = eng.grad_evals
     line = {"metric": "min-ESS/sec + grad evals/sec, stochastic volatility NUTS",
@@ -398,7 +411,8 @@ def bench_hier(args):
     C = G * 4
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), C)
     z0 = torch.zeros((C, model.dim), dtype=torch.float32, device="cuda")
-    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys)
+    eng, out, wall, warm_s, kms, kn = run_form(model, C, args.warmup, args.steps, z0, keys,
+                                               launch_shape=args.launch_shape)
     per = group_diagnostics(out["z"], model.dims, 4)
     mess, rhat = per["min_ess"], per["max_rhat"]
     ge = eng.grad_evals

@@ -255,8 +255,9 @@ int nax_exchange(nax_sampler* h, float* d_z_out, const double* d_pe_in, const fl
 int nax_set_positive_transform(nax_sampler* h, int32_t kind);
 
 /* The launch shapes the model's autotuner picked (waves per chain of a
- * persistent launch, row splits / kernel form of the logistic kernels, tile
- * rows of the hierarchical kernel, with the timings it compared), as text
+ * persistent launch, row splits / kernel form of the logistic kernels,
+ * chains per workgroup and waves per chain of the hierarchical kernel, with
+ * the timings it compared), as text
  * into buf (at most len bytes with the terminating zero; empty before the
  * first tuned launch).  No counterpart in the reference, which compiles one
  * XLA program per model. */
@@ -264,11 +265,12 @@ int nax_tuning(nax_sampler* h, char* buf, int32_t len);
 
 /* Fixes the launch shape the autotuner would pick: waves per chain (1, 2,
  * 4, 8, 16) of the per-chain persistent models (stochastic volatility, the
- * one-wave-per-chain logistic form), or chains per workgroup (a divisor of
- * chains_per_group) of the hierarchical model.  Draws depend on the waves
- * per chain (the order of its sums), not on the chains per workgroup; a
- * fixed shape makes runs repeat bit for bit.  No counterpart in the
- * reference. */
+ * one-wave-per-chain logistic form), or, for the hierarchical model, chains
+ * per workgroup kb (a divisor of chains_per_group) and waves per chain
+ * r (1, 2 or 4; kb r <= 16) as shape = kb + 32 (r - 1).  Draws depend on
+ * the waves per chain (the order of its sums), not on the chains per
+ * workgroup; a fixed shape makes runs repeat bit for bit.  No counterpart
+ * in the reference. */
 int nax_set_launch_shape(nax_sampler* h, int32_t shape);
 
 /* HMCState / post_warmup_state access (mcmc.py:558-587): copies the

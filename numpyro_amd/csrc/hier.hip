@@ -19,9 +19,15 @@
 // gradient the block evaluates all K together (chains that are already done
 // ride along and ignore the result).  Blocks take models heaviest first
 // (N_i D_i descending) to even out the tail of the launch.  A model's K
-// chains may also be split over K / KB workgroups of KB waves each (each
+// chains may also be split over K / KB workgroups of KB chains each (each
 // then reads X_i for its KB chains): more workgroups for the heavy models
-// against more passes over their X; the autotuner picks KB.
+// against more passes over their X.  And each chain may own R waves: the
+// block then stages R tiles (64 R rows) at a time, wave r of a chain takes
+// rows 64 r .. 64 r + 63 of them in both phases, and the R partial sums of
+// U and X^T R are added in r order at the end -- a heavy model's leaf then
+// takes ~1/R of the time, which is what the launch waits for.  The NUTS
+// state machine of a chain runs on its wave r = 0.  The autotuner picks
+// (KB, R).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,45 +65,54 @@ struct HierData {
 // column reading one row) and 16-byte aligned.
 constexpr int kHierLd = kHierMaxD + 4;
 
-template <int K>
+template <int K, int R>
 struct __attribute__((aligned(16))) HierShared {
-  __attribute__((aligned(16))) float xs[2][kHierRows][kHierLd];  // double-buffered X tile
+  __attribute__((aligned(16))) float xs[2][kHierRows * R][kHierLd];  // double-buffered X tiles
   __attribute__((aligned(16))) float beta[K][kHierMaxD];
-  __attribute__((aligned(16))) float res[K][kHierRows];  // residuals sigmoid(l) - y, chain-major
-  float ys[2][kHierRows];
+  __attribute__((aligned(16))) float res[K][kHierRows * R];  // residuals sigmoid(l) - y, chain-major
+  float ys[2][kHierRows * R];
+  // partial X^T R and U of the R waves of each chain (R > 1)
+  __attribute__((aligned(16))) float racc[R > 1 ? K : 1][R > 1 ? R : 1][kHierMaxD];
+  double rul[R > 1 ? K : 1][R > 1 ? R : 1];
 };
 
 // Tiles held in registers ahead of the one being processed: enough bytes in
 // flight per workgroup to cover the memory latency when a model's workgroup
 // runs alone on its CU (the tail of a launch), within ~64 VGPRs.
-template <int K>
+template <int K, int R>
 struct HierPipe {
-  static constexpr int NT = 64 * K;                       // threads
-  static constexpr int EPT = kHierRows * kHierMaxD / NT;  // tile elements per thread
+  static constexpr int NT = 64 * K * R;                       // threads
+  static constexpr int EPT = kHierRows * R * kHierMaxD / NT;  // tile elements per thread
   static constexpr int PF = (EPT >= 32) ? 1 : 2;
 };
 
 // Block-cooperative U and dU for the K chains of model i at z[c] (c = chain
-// index of the block's wave w: c = i K + w).  Wave w ends with its own
-// chain's results: returns U on every lane of wave w, writes dU to g[c].
-// Tiles of 64 rows are contiguous (rows * D floats) in the ragged layout.
-// Tile t is loaded into registers PF tiles ahead, staged into LDS buffer
-// t & 1, then: phase 1 (lane = row, wave = chain) logits and residuals,
-// phase 2 (lane = column, wave = chain) X^T R.  One barrier per tile: the
-// residuals a wave writes in phase 1 are read back only by that wave (LDS
-// accesses of one wavefront complete in order), and buffer t & 1 is next
-// written (tile t + 2) after every wave has passed the barrier of tile t + 1.
-template <int K>
+// index of the block's chain slot w: c = i K_model + sub K + w), each chain
+// on R waves (wave index w R + r).  Wave r = 0 of slot w ends with its own
+// chain's results: returns U on every lane of that wave (0 on the others),
+// writes dU to g[c].  Tiles of 64 R rows are contiguous (rows * D floats) in
+// the ragged layout.  Tile t is loaded into registers PF tiles ahead, staged
+// into LDS buffer t & 1, then: phase 1 (lane = row 64 r + lane, wave =
+// chain part) logits and residuals, phase 2 (lane = column, same wave) X^T R
+// over the same 64 rows.  One barrier per tile: the residuals a wave writes
+// in phase 1 are read back only by that wave (LDS accesses of one wavefront
+// complete in order), and buffer t & 1 is next written (tile t + 2) after
+// every wave has passed the barrier of tile t + 1.  With R > 1 the R
+// partial sums of X^T R and of U are added in r order after the last tile.
+template <int K, int R>
 __device__ double hier_group_eval(const HierData& hd, int i, int sub, const float* zreq,
-                                  int ldz, float* gout, int ldg, HierShared<K>& sh) {
-  using P = HierPipe<K>;
+                                  int ldz, float* gout, int ldg, HierShared<K, R>& sh) {
+  using P = HierPipe<K, R>;
   constexpr int NT = P::NT, EPT = P::EPT, PF = P::PF;
-  const int w = threadIdx.x >> 6;
+  constexpr int TR = kHierRows * R;  // rows of a staged tile
+  const int wa = threadIdx.x >> 6;
+  const int w = wa / R;               // chain slot
+  const int r = wa % R;               // 64-row part of the tile
   const int lane = threadIdx.x & 63;
   const int D = hd.dims[i];
   const int D4 = (D + 3) & ~3;
   const int64_t N = hd.row_off[i + 1] - hd.row_off[i];
-  const int ntiles = (int)((N + kHierRows - 1) / kHierRows);
+  const int ntiles = (int)((N + TR - 1) / TR);
   const float* X = hd.x + hd.elt_off[i];
   const float* Y = hd.y + hd.row_off[i];
   const int c = i * hd.K + sub * K + w;
@@ -107,11 +122,11 @@ __device__ double hier_group_eval(const HierData& hd, int i, int sub, const floa
   const PosTf tt = positive_tf((double)z[D + 1], hd.softplus != 0);
   const float tau = hd.softplus ? (float)tt.x : expf(z[D + 1]);
   const float eta = lane < D ? z[lane] : 0.0f;
-  sh.beta[w][lane] = lane < D ? mu + tau * eta : 0.0f;
+  if (r == 0) sh.beta[w][lane] = lane < D ? mu + tau * eta : 0.0f;
   // columns [D, D4) of both buffers stay zero for the whole evaluation
-  for (int e = threadIdx.x; e < 2 * kHierRows * 4; e += NT) {
-    const int b = e / (kHierRows * 4), r = (e / 4) % kHierRows, cc = D + (e & 3);
-    if (cc < D4) sh.xs[b][r][cc] = 0.0f;
+  for (int e = threadIdx.x; e < 2 * TR * 4; e += NT) {
+    const int b = e / (TR * 4), rr = (e / 4) % TR, cc = D + (e & 3);
+    if (cc < D4) sh.xs[b][rr][cc] = 0.0f;
   }
   // element e = threadIdx.x + NT j of a tile sits at row e / D, column e % D
   const int step_r = NT / D, step_c = NT % D;
@@ -119,8 +134,8 @@ __device__ double hier_group_eval(const HierData& hd, int i, int sub, const floa
   float px[PF][EPT];
   float py[PF];
   auto load = [&](int t, float* dst, float& ydst) {
-    const int64_t r0 = (int64_t)t * kHierRows;
-    const int rows = (int)min<int64_t>(kHierRows, N - r0);
+    const int64_t r0 = (int64_t)t * TR;
+    const int rows = (int)min<int64_t>(TR, N - r0);
     const int ne = rows * D;
     const float* src = X + r0 * D;
 #pragma unroll
@@ -133,34 +148,36 @@ __device__ double hier_group_eval(const HierData& hd, int i, int sub, const floa
 #pragma unroll
   for (int j = 0; j < PF; ++j)
     if (j < ntiles) load(j, px[j], py[j]);
-  float acc = 0.0f;   // phase 2: dU/dbeta_lane for chain w
-  double ul = 0.0;    // phase 1: likelihood part of U for chain w
+  const int rr0 = kHierRows * r;  // this wave's first row in a tile
+  float acc = 0.0f;   // phase 2: dU/dbeta_lane for chain w over this wave's rows
+  double ul = 0.0;    // phase 1: likelihood part of U for chain w over this wave's rows
   for (int t0 = 0; t0 < ntiles; t0 += PF) {
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
       const int t = t0 + j;
       if (t < ntiles) {
-        const int rows = (int)min<int64_t>(kHierRows, N - (int64_t)t * kHierRows);
+        const int rows = (int)min<int64_t>(TR, N - (int64_t)t * TR);
         float(*xs)[kHierLd] = sh.xs[t & 1];
         {
-          // the whole 64-row tile is written: rows past `rows` get zeros
-          const int ne = kHierRows * D;
-          int r = r_first, cc = c_first;
+          // the whole tile is written: rows past `rows` get zeros
+          const int ne = TR * D;
+          int rr = r_first, cc = c_first;
 #pragma unroll
           for (int q = 0; q < EPT; ++q) {
-            if (threadIdx.x + NT * q < ne) xs[r][cc] = px[j][q];
-            r += step_r;
+            if (threadIdx.x + NT * q < ne) xs[rr][cc] = px[j][q];
+            rr += step_r;
             cc += step_c;
-            if (cc >= D) { cc -= D; r += 1; }
+            if (cc >= D) { cc -= D; rr += 1; }
           }
-          if (threadIdx.x < kHierRows) sh.ys[t & 1][threadIdx.x] = py[j];
+          if (threadIdx.x < TR) sh.ys[t & 1][threadIdx.x] = py[j];
         }
         if (t + PF < ntiles) load(t + PF, px[j], py[j]);  // in flight during the compute below
         __syncthreads();
+        const int rows_r = min(max(rows - rr0, 0), kHierRows);
         float rv = 0.0f;
-        if (lane < rows) {
+        if (lane < rows_r) {
           float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
-          const float4* xr = reinterpret_cast<const float4*>(&xs[lane][0]);
+          const float4* xr = reinterpret_cast<const float4*>(&xs[rr0 + lane][0]);
           const float4* br = reinterpret_cast<const float4*>(&sh.beta[w][0]);
 #pragma unroll
           for (int q = 0; q < kHierMaxD / 4; ++q) {
@@ -173,35 +190,53 @@ __device__ double hier_group_eval(const HierData& hd, int i, int sub, const floa
             }
           }
           float u;
-          logistic_terms_f((l0 + l1) + (l2 + l3), sh.ys[t & 1][lane], rv, u);
+          logistic_terms_f((l0 + l1) + (l2 + l3), sh.ys[t & 1][rr0 + lane], rv, u);
           ul += (double)u;
         }
-        sh.res[w][lane] = rv;
+        sh.res[w][rr0 + lane] = rv;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < D) {
-          const int rows4 = (rows + 3) & ~3;  // residuals of rows past `rows` are 0
-          const float4* rr = reinterpret_cast<const float4*>(&sh.res[w][0]);
+          const int rows4 = (rows_r + 3) & ~3;  // residuals of rows past `rows` are 0
+          const float4* rq = reinterpret_cast<const float4*>(&sh.res[w][rr0]);
           float a0 = 0.0f, a1 = 0.0f;
           for (int q = 0; q < rows4 / 4; ++q) {
-            const float4 rv4 = rr[q];
-            a0 = fmaf(xs[4 * q][lane], rv4.x, a0);
-            a1 = fmaf(xs[4 * q + 1][lane], rv4.y, a1);
-            a0 = fmaf(xs[4 * q + 2][lane], rv4.z, a0);
-            a1 = fmaf(xs[4 * q + 3][lane], rv4.w, a1);
+            const float4 rv4 = rq[q];
+            a0 = fmaf(xs[rr0 + 4 * q][lane], rv4.x, a0);
+            a1 = fmaf(xs[rr0 + 4 * q + 1][lane], rv4.y, a1);
+            a0 = fmaf(xs[rr0 + 4 * q + 2][lane], rv4.z, a0);
+            a1 = fmaf(xs[rr0 + 4 * q + 3][lane], rv4.w, a1);
           }
           acc += a0 + a1;
         }
       }
     }
   }
+  double U_lik;
+  if constexpr (R > 1) {
+    sh.racc[w][r][lane] = acc;
+    const double uw = wave_sum(ul);
+    if (lane == 0) sh.rul[w][r] = uw;
+    __syncthreads();
+    if (r != 0) return 0.0;
+    float a = 0.0f;
+    double us = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      a += sh.racc[w][q][lane];
+      us += sh.rul[w][q];
+    }
+    acc = a;
+    U_lik = us;
+  } else {
+    U_lik = wave_sum(ul);
+  }
   // chain w's own gradient: dU/deta = eta + tau acc, dU/dmu = mu + sum acc,
   // dU/dlog tau = tau eta.acc + 2 tau^2 / (1 + tau^2) - 1
   const double sum_acc = wave_sum((double)acc);
   const double eta_acc = wave_sum((double)eta * acc);
   const double eta2 = wave_sum((double)eta * eta);
-  const double U_lik = wave_sum(ul);
   if (lane < D) g[lane] = eta + tau * acc;
   const double t2 = (double)tau * tau;
   if (lane == 0) {
@@ -222,38 +257,43 @@ template <int K>
 __global__ void __launch_bounds__(64 * K)
 hier_eval_kernel(HierData hd, const float* __restrict__ zreq, int ldz, double* __restrict__ pe,
                  float* __restrict__ gout, int ldg) {
-  __shared__ HierShared<K> sh;
+  __shared__ HierShared<K, 1> sh;
   const int per = hd.K / K;  // workgroups per model
   const int i = hd.order[blockIdx.x / per];
   const int sub = blockIdx.x % per;
-  const double u = hier_group_eval<K>(hd, i, sub, zreq, ldz, gout, ldg, sh);
+  const double u = hier_group_eval<K, 1>(hd, i, sub, zreq, ldz, gout, ldg, sh);
   if ((threadIdx.x & 63) == 0) pe[i * hd.K + sub * K + (threadIdx.x >> 6)] = u;
 }
 
-template <int K, bool DENSE>
-__global__ void __launch_bounds__(64 * K, 2)
+// One block per (model, chain subset): K chain slots of R waves each.  Wave
+// r = 0 of a slot runs its chain's NUTS state machine; every wave of the
+// block joins each gradient evaluation.
+template <int K, int R, bool DENSE>
+__global__ void __launch_bounds__(64 * K * R, (K * R <= 8) ? 2 : 1)
 hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col,
                        const int64_t* __restrict__ d_target, ChainState* __restrict__ states,
                        float* __restrict__ vec, double* __restrict__ welf,
                        float* __restrict__ zreq, float* __restrict__ gout,
                        double* __restrict__ peout, int64_t* __restrict__ grad_evals) {
-  __shared__ HierShared<K> sh;
+  __shared__ HierShared<K, R> sh;
   __shared__ int need_any;
   __shared__ ChainState sS[K];
   const int64_t target = *d_target;
   const int per = hd.K / K;  // workgroups per model
   const int i = hd.order[blockIdx.x / per];
   const int sub = blockIdx.x % per;
-  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wa = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int w = wa / R;
+  const bool head = wa % R == 0;  // the wave that runs the state machine
   const int c = i * hd.K + sub * K + w;
   const ChainCtx cx = make_ctx<DENSE>(cfg, c, vec, welf);
   float* zr = zreq + (size_t)c * cfg.Dp;
   float* gi = gout + (size_t)c * cfg.Dp;
   ChainState& S = sS[w];
-  copy_state(S, states[c], cx.lane);
+  if (head) copy_state(S, states[c], cx.lane);
   double pe = peout[c];
   int evals = 0;
-  bool active = true;
+  bool active = head;
   for (;;) {
     if (threadIdx.x == 0) need_any = 0;
     __syncthreads();
@@ -266,14 +306,16 @@ hier_persistent_kernel(HierData hd, EngineCfg cfg, Collect col,
     __threadfence_block();
     __syncthreads();
     if (!need_any) break;
-    const double u = hier_group_eval<K>(hd, i, sub, zreq, cfg.Dp, gout, cfg.Dp, sh);
+    const double u = hier_group_eval<K, R>(hd, i, sub, zreq, cfg.Dp, gout, cfg.Dp, sh);
     if (need) pe = u;
     __threadfence_block();
   }
-  copy_state(states[c], S, cx.lane);
-  if (cx.lane == 0) {
-    peout[c] = pe;
-    if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
+  if (head) {
+    copy_state(states[c], S, cx.lane);
+    if (cx.lane == 0) {
+      peout[c] = pe;
+      if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
+    }
   }
 }
 
@@ -306,15 +348,32 @@ struct HierPotential : Potential {
   double alg_bytes(int) const override { return bytes; }
   double alg_flops(int) const override { return flops; }
 
-  int kb = 0;  // waves (chains) per workgroup, tuned; 0: hd.K
+  // launch shape: kb chains per workgroup (a divisor of hd.K; 0: hd.K) and
+  // rw waves per chain, tuned or fixed; code = kb + 32 (rw - 1)
+  int kb = 0, rw = 1;
   int tuned_chains = 0;
   bool forced = false;
+  static int shape_code(int k, int r) { return k + 32 * (r - 1); }
+  static std::string shape_label(int code) {
+    return std::to_string(code % 32) + "x" + std::to_string(code / 32 + 1);
+  }
+  // the (kb, rw) pairs with a kernel: rw in {1, 2, 4}, kb rw <= 16
+  static bool has_kernel(int k, int r) {
+    return (r == 1 && (k == 1 || k == 2 || k == 4 || k == 8 || k == 16)) ||
+           ((r == 2 || r == 4) && (k == 1 || k == 2 || k == 4) && k * r <= 16);
+  }
+  void describe(const std::string& how) {
+    tuning = "hier: " + std::to_string(waves()) + " chains per workgroup of " +
+             std::to_string(hd.K) + " per model, " + std::to_string(rw) +
+             " waves per chain " + how;
+  }
   bool force_launch_shape(int shape) override {
-    if (shape < 1 || shape > hd.K || hd.K % shape) return false;
-    kb = shape;
+    const int k = shape % 32, r = shape / 32 + 1;
+    if (k < 1 || k > hd.K || hd.K % k || !has_kernel(k, r)) return false;
+    kb = k;
+    rw = r;
     forced = true;
-    tuning = "hier: " + std::to_string(kb) + " chains per workgroup of " +
-             std::to_string(hd.K) + " per model (fixed)";
+    describe("(fixed)");
     return true;
   }
 
@@ -336,44 +395,63 @@ struct HierPotential : Potential {
     return hipGetLastError();
   }
 
-  hipError_t launch(int w, hipStream_t s, const EngineCfg& cfg, const Collect& col,
+  hipError_t launch(int code, hipStream_t s, const EngineCfg& cfg, const Collect& col,
                     const int64_t* d_target, ChainState* st, float* vec, double* wel, float* zreq,
                     float* gout, double* peout, int64_t* gevals) const {
+    const int w = code % 32, r = code / 32 + 1;
+    if (w < 1 || hd.K % w) return hipErrorInvalidValue;
     const dim3 grid(groups * (hd.K / w));
-    switch (w) {
-#define NAX_K(KV)                                                                          \
-  case KV:                                                                                 \
-    if (cfg.dense)                                                                         \
-      hipLaunchKernelGGL((hier_persistent_kernel<KV, true>), grid, dim3(64 * KV), 0, s, hd, \
-                         cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals);     \
-    else                                                                                   \
-      hipLaunchKernelGGL((hier_persistent_kernel<KV, false>), grid, dim3(64 * KV), 0, s,    \
-                         hd, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals); \
-    break;
-      NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
-#undef NAX_K
-      default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+#define NAX_KR(KV, RV)                                                                        \
+  if (w == KV && r == RV) {                                                                   \
+    if (cfg.dense)                                                                            \
+      hipLaunchKernelGGL((hier_persistent_kernel<KV, RV, true>), grid, dim3(64 * KV * RV), 0,  \
+                         s, hd, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals); \
+    else                                                                                      \
+      hipLaunchKernelGGL((hier_persistent_kernel<KV, RV, false>), grid, dim3(64 * KV * RV), 0, \
+                         s, hd, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals); \
+    return hipGetLastError();                                                                 \
+  }
+    NAX_KR(1, 1) NAX_KR(2, 1) NAX_KR(4, 1) NAX_KR(8, 1) NAX_KR(16, 1)
+    NAX_KR(1, 2) NAX_KR(2, 2) NAX_KR(4, 2)
+    NAX_KR(1, 4) NAX_KR(2, 4) NAX_KR(4, 4)
+#undef NAX_KR
+    return hipErrorInvalidValue;
   }
 
   hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
                                 double* wel, float* zreq, float* gout, double* peout) override {
     // chains per workgroup: K (one workgroup per model) or K/2, K/4 (each
-    // model's chains split over 2 or 4 workgroups), timed over a few
-    // iterations from a scratch copy of the state
+    // model's chains split over 2 or 4 workgroups); waves per chain 1, 2 or
+    // 4; timed over a few iterations from a scratch copy of the state.  The
+    // choice is kept per process for this shape of workload (the waves per
+    // chain set the order of the sums, so the draws).
     if (forced || (kb > 0 && tuned_chains == cfg.C)) return hipSuccess;
+    const std::string key = "hier G=" + std::to_string(groups) + " K=" + std::to_string(hd.K) +
+                            " C=" + std::to_string(cfg.C) + " dense=" + std::to_string(cfg.dense);
+    int code = 0;
+    if (tuned_shape(key, &code, false)) {
+      kb = code % 32;
+      rw = code / 32 + 1;
+      tuned_chains = cfg.C;
+      describe("(tuned earlier in this process)");
+      return hipSuccess;
+    }
     std::vector<int> cands;
-    for (int w = hd.K; w >= 1 && (int)cands.size() < 3; w /= 2) cands.push_back(w);
+    for (int w = hd.K; w >= 1 && w * 4 >= hd.K; w /= 2)
+      for (int r : {1, 2, 4})
+        if (has_kernel(w, r)) cands.push_back(shape_code(w, r));
     std::string log;
-    auto go = [&](int w, hipStream_t ss, const Collect& col, const int64_t* tgt, ChainState* st2,
+    auto go = [&](int c, hipStream_t ss, const Collect& col, const int64_t* tgt, ChainState* st2,
                   float* vec2, double* wel2, float* zr2, float* g2, double* pe2, int64_t* ge2) {
-      return launch(w, ss, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2, ge2);
+      return launch(c, ss, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2, ge2);
     };
-    kb = tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, 3, cands, go, &log);
+    code = tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, 3, cands, go, &log,
+                           &HierPotential::shape_label);
+    kb = code % 32;
+    rw = code / 32 + 1;
+    tuned_shape(key, &code, true);
     tuned_chains = cfg.C;
-    tuning = "hier: " + std::to_string(kb) + " chains per workgroup of " +
-             std::to_string(hd.K) + " per model (ms for 3 iterations:" + log + ")";
+    describe("(ms for 3 iterations, chains per workgroup x waves per chain:" + log + ")");
     return hipSuccess;
   }
 
@@ -381,8 +459,8 @@ struct HierPotential : Potential {
                             const int64_t* d_target, ChainState* st, float* vec, double* wel,
                             float* zreq, float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    const hipError_t e = launch(waves(), s, cfg, col, d_target, st, vec, wel, zreq, gout, peout,
-                                gevals);
+    const hipError_t e = launch(shape_code(waves(), rw), s, cfg, col, d_target, st, vec, wel,
+                                zreq, gout, peout, gevals);
     mark_end(s);
     return e;
   }

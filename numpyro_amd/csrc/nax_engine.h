@@ -1055,11 +1055,13 @@ hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineC
 // (launch(cand, stream, Collect, d_target, st, vec, wel, zreq, gout, peout,
 // gevals)) for `iters` iterations -- the same NUTS work, nothing collected --
 // timed with HIP events; the fastest candidate is returned.  The sampler's
-// own state is not touched.  `log` receives "candidate:ms" for each.
+// own state is not touched.  `log` receives "candidate:ms" for each (the
+// candidate written by `label` when given).
 template <class Launch>
 int tune_persistent(hipStream_t s, const EngineCfg& cfg, const ChainState* st, const float* vec,
                     const double* wel, const float* zreq, const float* gout, const double* peout,
-                    int iters, const std::vector<int>& cands, Launch launch, std::string* log) {
+                    int iters, const std::vector<int>& cands, Launch launch, std::string* log,
+                    std::string (*label)(int) = nullptr) {
   if (cands.size() < 2) return cands.empty() ? 1 : cands[0];
   const size_t nst = sizeof(ChainState) * cfg.C;
   const size_t nvec = sizeof(float) * (size_t)cfg.C * cfg.nvec * cfg.Dp;
@@ -1100,7 +1102,7 @@ int tune_persistent(hipStream_t s, const EngineCfg& cfg, const ChainState* st, c
           launch(w, s, none, ge2 + 1, st2, vec2, wel2, zr2, g2, pe2, ge2) == hipSuccess &&
           hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess)
         hipEventElapsedTime(&ms, e0, e1);
-      if (log) *log += " " + std::to_string(w) + ":" + std::to_string(ms);
+      if (log) *log += " " + (label ? label(w) : std::to_string(w)) + ":" + std::to_string(ms);
       if (ms < best_ms) {
         best_ms = ms;
         best = w;

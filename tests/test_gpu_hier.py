@@ -163,7 +163,11 @@ def test_hier_softplus_potential_grad():
 def test_hier_graph_replay_matches_direct_launches_and_reports_tuning():
     # nax_run replays the persistent chunks (target step + launch) as one
     # captured hipGraph; direct launches must give every draw bit for bit.
-    # The autotuned chains-per-workgroup split does not change any draw.
+    # The launch shape is (chains per workgroup, waves per chain), code
+    # kb + 32 (waves - 1).  The chains-per-workgroup split does not change
+    # any draw; the waves per chain set the order of the sums, so runs
+    # compare bit for bit within one waves-per-chain setting, and the
+    # autotuned shape is kept for the process, so two tuned engines agree.
     from numpyro_amd import random as nrandom
     from numpyro_amd.infer.hmc import Engine
     from numpyro_amd.models import HierarchicalLogistic
@@ -172,8 +176,9 @@ def test_hier_graph_replay_matches_direct_launches_and_reports_tuning():
     K = 4
     C = len(shapes) * K
     keys = nrandom.chain_init_keys(nrandom.PRNGKey(4), C)
-    outs = []
-    for graphs, shape in ((True, None), (False, None), (False, 1), (True, 2)):
+    runs = {}
+    for graphs, shape in ((True, None), (False, None), (False, 1), (True, 2), (False, 4),
+                          (False, 36), (True, 34), (True, 97), (False, 100)):
         eng = Engine(HierarchicalLogistic(Xs, ys, chains_per_group=K), C, "NUTS", 60)
         eng.set_graphs(graphs)
         if shape:
@@ -183,7 +188,57 @@ def test_hier_graph_replay_matches_direct_launches_and_reports_tuning():
         tun = eng.tuning()
         assert tun.startswith("hier: ") and "chains per workgroup" in tun, tun
         print(f"[hier graphs={graphs} shape={shape}] {tun}", flush=True)
-        outs.append({k: v.cpu().numpy() for k, v in out.items()})
-    for o in outs[1:]:
-        for k in outs[0]:
-            assert np.array_equal(outs[0][k], o[k]), k
+        runs[(graphs, shape)] = {k: v.cpu().numpy() for k, v in out.items()}
+    groups = [[(True, None), (False, None)], [(False, 1), (True, 2), (False, 4)],
+              [(False, 36), (True, 34)], [(True, 97), (False, 100)]]
+    for grp in groups:
+        for other in grp[1:]:
+            for k in runs[grp[0]]:
+                assert np.array_equal(runs[grp[0]][k], runs[other][k]), (grp[0], other, k)
+    # 1, 2 and 4 waves per chain: the same chains up to float rounding of
+    # the sums -- equal tree sizes over the first iterations
+    for shape in (36, 100):
+        assert np.array_equal(runs[(False, 4)]["num_steps"][:, :5],
+                              runs[(False, shape)]["num_steps"][:, :5]), shape
+        np.testing.assert_allclose(runs[(False, 4)]["z"][:, :3], runs[(False, shape)]["z"][:, :3],
+                                   rtol=1e-3, atol=1e-4)
+
+
+def test_hier_warmup_iterations_follow_oracle_with_batch_keys():
+    """One model of the 1024-model workload (make_hier(1024, 4), model 593,
+    N=376, D=33) run alone with the keys its chains get in the batch
+    (chain_keys(PRNGKey(0), 4096)[4g:4g+4]), from z = 0 with dual averaging
+    and diagonal Welford active (500 warmup): the first 8 trees have the
+    oracle's sizes and step sizes and the first 6 positions agree within
+    1e-3.  Later iterations part ways as fp32 and fp64 round differently
+    and an accept decision flips, as any two float implementations do."""
+    from numpyro_amd import random as nrandom
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import HierarchicalLogistic
+    from numpyro_amd.workloads import make_hier
+    g, K, T = 593, 8, 6
+    Xs, ys = make_hier(1024, 4)
+    X, y = Xs[g], ys[g]
+    D = X.shape[1]
+    keys = nrandom.chain_init_keys(nrandom.PRNGKey(0), 4096)[4 * g:4 * g + 4]
+    np.testing.assert_array_equal(keys, np.asarray(ohmc.chain_keys(ojr.PRNGKey(0), 4096))[4 * g:4 * g + 4])
+    eng = Engine(HierarchicalLogistic([X], [y], chains_per_group=4), 4, "NUTS", 500)
+    eng.init(torch.zeros((4, eng.D)), np.asarray(keys, np.uint32))
+    out = eng.run(K, collect_fields=("z", "num_steps", "adapt_state.step_size"), start_idx=0,
+                  size=K)
+    gz = out["z"].cpu().numpy().astype(np.float64)[..., :D + 2]
+    gns = out["num_steps"].cpu().numpy()
+    gss = out["adapt_state.step_size"].cpu().numpy()
+    f = lambda q: om.hier_potential_grad(q, X.astype(np.float64), y.astype(np.float64))
+    for c in range(4):
+        ik, sk = ohmc.hmc(f)
+        st = ik(np.zeros(D + 2), 500, rng_key=keys[c])
+        ns, zs, ss = [], [], []
+        for _ in range(K):
+            st = sk(st)
+            ns.append(st.num_steps)
+            zs.append(st.z)
+            ss.append(st.adapt_state.step_size)
+        assert list(gns[c]) == ns, (c, list(gns[c]), ns)
+        np.testing.assert_allclose(gss[c], ss, rtol=1e-3)
+        np.testing.assert_allclose(gz[c, :T], np.stack(zs)[:T], rtol=1e-3, atol=1e-3)
