@@ -349,10 +349,17 @@ struct HierPotential : Potential {
   double alg_flops(int) const override { return flops; }
 
   // launch shape: kb chains per workgroup (a divisor of hd.K; 0: hd.K) and
-  // rw waves per chain, tuned or fixed; code = kb + 32 (rw - 1)
+  // rw waves per chain, code = kb + 32 (rw - 1).  The default (hd.K, 1) runs
+  // until iteration kHierTuneAt, where the tuned shape (kb_t, rw_t) takes
+  // over -- timed on trees of working size, not the first iterations' few
+  // leaves -- unless a shape was fixed.
+  static constexpr int64_t kHierTuneAt = 50;
   int kb = 0, rw = 1;
+  int kb_t = 0, rw_t = 1;
+  std::string tuned_log;
   int tuned_chains = 0;
   bool forced = false;
+  HierPotential() { tune_at = kHierTuneAt; }
   static int shape_code(int k, int r) { return k + 32 * (r - 1); }
   static std::string shape_label(int code) {
     return std::to_string(code % 32) + "x" + std::to_string(code / 32 + 1);
@@ -373,6 +380,7 @@ struct HierPotential : Potential {
     kb = k;
     rw = r;
     forced = true;
+    tune_at = 0;
     describe("(fixed)");
     return true;
   }
@@ -420,38 +428,49 @@ struct HierPotential : Potential {
 
   hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
                                 double* wel, float* zreq, float* gout, double* peout) override {
-    // chains per workgroup: K (one workgroup per model) or K/2, K/4 (each
-    // model's chains split over 2 or 4 workgroups); waves per chain 1, 2 or
-    // 4; timed over a few iterations from a scratch copy of the state.  The
-    // choice is kept per process for this shape of workload (the waves per
-    // chain set the order of the sums, so the draws).
-    if (forced || (kb > 0 && tuned_chains == cfg.C)) return hipSuccess;
-    const std::string key = "hier G=" + std::to_string(groups) + " K=" + std::to_string(hd.K) +
-                            " C=" + std::to_string(cfg.C) + " dense=" + std::to_string(cfg.dense);
-    int code = 0;
-    if (tuned_shape(key, &code, false)) {
-      kb = code % 32;
-      rw = code / 32 + 1;
-      tuned_chains = cfg.C;
-      describe("(tuned earlier in this process)");
+    // candidates: one workgroup per model (kb = K) with 1, 2 or 4 waves per
+    // chain, or each model's chains split over 2 or 4 workgroups (K/2, K/4)
+    // with one wave each; timed over 5 iterations from a scratch copy of
+    // the state.  The choice is kept per process for this shape of workload
+    // (the waves per chain set the order of the sums, so the draws).
+    if (forced) return hipSuccess;
+    if (chain_iter < tune_at) {
+      kb = 0;
+      rw = 1;
+      describe("(default until iteration " + std::to_string(tune_at) + ", then tuned)");
       return hipSuccess;
     }
-    std::vector<int> cands;
-    for (int w = hd.K; w >= 1 && w * 4 >= hd.K; w /= 2)
-      for (int r : {1, 2, 4})
-        if (has_kernel(w, r)) cands.push_back(shape_code(w, r));
-    std::string log;
-    auto go = [&](int c, hipStream_t ss, const Collect& col, const int64_t* tgt, ChainState* st2,
-                  float* vec2, double* wel2, float* zr2, float* g2, double* pe2, int64_t* ge2) {
-      return launch(c, ss, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2, ge2);
-    };
-    code = tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, 3, cands, go, &log,
-                           &HierPotential::shape_label);
-    kb = code % 32;
-    rw = code / 32 + 1;
-    tuned_shape(key, &code, true);
-    tuned_chains = cfg.C;
-    describe("(ms for 3 iterations, chains per workgroup x waves per chain:" + log + ")");
+    if (kb_t == 0 || tuned_chains != cfg.C) {
+      const std::string key = "hier G=" + std::to_string(groups) + " K=" + std::to_string(hd.K) +
+                              " C=" + std::to_string(cfg.C) + " dense=" + std::to_string(cfg.dense);
+      int code = 0;
+      if (tuned_shape(key, &code, false)) {
+        tuned_log = "(tuned earlier in this process)";
+      } else {
+        std::vector<int> cands;
+        for (int r : {1, 2, 4})
+          if (has_kernel(hd.K, r)) cands.push_back(shape_code(hd.K, r));
+        for (int w = hd.K / 2; w >= 1 && w * 4 >= hd.K; w /= 2)
+          if (has_kernel(w, 1)) cands.push_back(shape_code(w, 1));
+        std::string log;
+        auto go = [&](int c, hipStream_t ss, const Collect& col, const int64_t* tgt,
+                      ChainState* st2, float* vec2, double* wel2, float* zr2, float* g2,
+                      double* pe2, int64_t* ge2) {
+          return launch(c, ss, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2, ge2);
+        };
+        code = tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, 5, cands, go, &log,
+                               &HierPotential::shape_label);
+        tuned_shape(key, &code, true);
+        tuned_log = "(from iteration " + std::to_string(tune_at) +
+                    "; ms for 5 iterations, chains per workgroup x waves per chain:" + log + ")";
+      }
+      kb_t = code % 32;
+      rw_t = code / 32 + 1;
+      tuned_chains = cfg.C;
+    }
+    kb = kb_t;
+    rw = rw_t;
+    describe(tuned_log);
     return hipSuccess;
   }
 

@@ -126,6 +126,13 @@ struct Potential {
                                         double* peout) {
     return hipSuccess;
   }
+  // A model whose launch shape is best chosen on trees of working size sets
+  // tune_at > 0: chains run its default shape below that iteration, the
+  // sampler ends a chunk exactly there and calls prepare_persistent again
+  // (chain_iter holds the chains' iteration at each call), so every run of
+  // the same keys uses the same shape at the same iterations.
+  int64_t tune_at = 0;
+  int64_t chain_iter = 0;
   // HMCECS models: one Gibbs update of every chain's subsample at its
   // current position (z, stride ldv), accepting into st[c].pe and the state
   // gradient gst (stride ldv); accept[c] receives the acceptance probability.

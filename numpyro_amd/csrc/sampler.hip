@@ -684,24 +684,26 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
     // kMaxChunk iterations).  The iteration target lives in device memory
     // (gevals[1]): a chunk is a target step kernel plus the persistent
     // launch, enqueued directly or replayed as a hipGraph captured for the
-    // current chunk length (the model's autotuning runs before, uncaptured).
-    // Kernel timing brackets the main kernel with events, so it runs
-    // uncaptured.
+    // current chunk length (the model's autotuning runs before, uncaptured;
+    // a model that tunes at iteration tune_at has a chunk end there and is
+    // prepared again).  Kernel timing brackets the main kernel with events,
+    // so it runs uncaptured.
     constexpr int64_t kPersistIters = 50, kMaxChunk = 1000;
     constexpr double kChunkMs = 1000.0;
     std::vector<ChainState> probe(1);
     NAX_HIP(hipMemcpyAsync(probe.data(), h->st, sizeof(ChainState), hipMemcpyDeviceToHost, h->stream));
     NAX_HIP(hipStreamSynchronize(h->stream));
     int64_t* d_target = h->gevals + 1;
+    int64_t t = probe[0].i;
+    h->pot->chain_iter = t;
     NAX_HIP(h->pot->prepare_persistent(h->stream, h->cfg, h->st, h->vec, h->wel, h->zreq,
                                        h->gout, h->peout));
-    int64_t t = probe[0].i;
     hipLaunchKernelGGL(set_target_kernel, dim3(1), dim3(1), 0, h->stream, d_target, t);
     NAX_HIP(hipGetLastError());
     int64_t step = kPersistIters;
-    auto chunk = [&]() -> hipError_t {
+    auto chunk = [&](int64_t len) -> hipError_t {
       hipLaunchKernelGGL(step_target_kernel, dim3(1), dim3(1), 0, h->stream, d_target, target,
-                         step);
+                         len);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       return h->pot->run_persistent(h->stream, h->cfg, col, d_target, h->st, h->vec, h->wel,
@@ -715,13 +717,18 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
     } guard{&exec};
     int64_t launches = 0;
     while (t < target) {
-      t = std::min<int64_t>(target, t + step);
+      // a chunk ends at the model's tuning iteration (run directly: its
+      // length differs from the captured one)
+      const int64_t ta = h->pot->tune_at;
+      const int64_t len = t < ta ? std::min<int64_t>(step, ta - t) : step;
+      const int64_t t0 = t;
+      t = std::min<int64_t>(target, t + len);
       const int k = (int)(launches % kBatch);
       const auto wall0 = std::chrono::steady_clock::now();
-      if (use_graph) {
+      if (use_graph && len == step) {
         if (!exec) {
           NAX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-          const hipError_t le = chunk();
+          const hipError_t le = chunk(step);
           hipGraph_t graph = nullptr;
           const hipError_t ce = hipStreamEndCapture(h->stream, &graph);
           if (le != hipSuccess || ce != hipSuccess) {
@@ -733,10 +740,12 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
           NAX_HIP(ie);
         }
         NAX_HIP(hipGraphLaunch(exec, h->stream));
+      } else if (use_graph) {
+        NAX_HIP(chunk(len));
       } else {
         int rc = eval_events(h, k);
         if (rc) return rc;
-        NAX_HIP(chunk());
+        NAX_HIP(chunk(len));
         h->pot->ev_begin = h->pot->ev_end = nullptr;
       }
       NAX_HIP(hipStreamSynchronize(h->stream));
@@ -748,7 +757,7 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
         const double ms = std::chrono::duration<double, std::milli>(
                               std::chrono::steady_clock::now() - wall0).count();
         const int64_t next = std::max<int64_t>(
-            kPersistIters, std::min<int64_t>(kMaxChunk, (int64_t)(step * kChunkMs /
+            kPersistIters, std::min<int64_t>(kMaxChunk, (int64_t)(len * kChunkMs /
                                                                    std::max(ms, 1e-3))));
         if (next != step) {
           step = next;
@@ -756,6 +765,16 @@ extern "C" int nax_run(nax_sampler* h, int64_t target, const nax_collect* c, int
             hipGraphExecDestroy(exec);
             exec = nullptr;
           }
+        }
+      }
+      if (t0 < ta && t >= ta && t < target) {
+        h->pot->chain_iter = t;
+        const std::string before = h->pot->tuning;
+        NAX_HIP(h->pot->prepare_persistent(h->stream, h->cfg, h->st, h->vec, h->wel, h->zreq,
+                                           h->gout, h->peout));
+        if (exec && h->pot->tuning != before) {  // captured with the old launch shape
+          hipGraphExecDestroy(exec);
+          exec = nullptr;
         }
       }
       ++launches;
