@@ -1,4 +1,4 @@
-"""Synthetic logistic-regression workloads (generated on the GPU).
+"""Synthetic logistic-regression workloads.
 
 Shapes follow the benchmark configurations:
   c1: X in R^{1000 x 25} N(0,1) + intercept, beta* ~ N(0,1); 4 chains.
@@ -6,10 +6,13 @@ Shapes follow the benchmark configurations:
       intercept, beta* ~ N(0, 0.5^2); 256 chains.
   c5: N=2,000,000, D=256, X ~ N(0,1) with 51 (20%) columns scaled x10,
       beta* ~ N(0, 0.1^2); 1024 chains.
-y ~ Bernoulli(sigmoid(X beta*)).  The draws use torch's generator on the
-device, seeded per configuration, so a 2M x 256 design is built in HBM in
-well under a second.
+y ~ Bernoulli(sigmoid(X beta*)).  The draws come from numpy's PCG64
+(``default_rng(seed)``, one seed per configuration), in float64 and in the
+order listed, X then cast to float32; the logits are computed in float64
+from the float32 X.  ``design`` returns the numpy arrays, so a CPU checker
+regenerates a design bit for bit; ``make`` moves them to the GPU.
 """
+import numpy as np
 import torch
 
 CONFIGS = {
@@ -18,42 +21,64 @@ CONFIGS = {
     "c5": dict(N=2_000_000, D=256, chains=1024, seed=5),
 }
 
+_ROWS = 1 << 17  # rows per generation chunk (float64 scratch of 268 MB at D=256)
 
-def make(name, device="cuda", N=None, chains=None):
+
+def _normal_rows(rng, N, D):
+    X = np.empty((N, D), np.float32)
+    for r0 in range(0, N, _ROWS):
+        X[r0:r0 + _ROWS] = rng.standard_normal((min(_ROWS, N - r0), D))
+    return X
+
+
+def _labels(rng, X, beta):
+    N = X.shape[0]
+    logits = np.empty(N)
+    for r0 in range(0, N, _ROWS):
+        logits[r0:r0 + _ROWS] = X[r0:r0 + _ROWS].astype(np.float64) @ beta
+    return (rng.uniform(size=N) < 1.0 / (1.0 + np.exp(-logits))).astype(np.float32)
+
+
+def design(name, N=None):
+    """(X float32 [N, D], y float32 [N], beta* float64 [D], cfg) as numpy
+    arrays, drawn from default_rng(cfg['seed'])."""
     cfg = dict(CONFIGS[name])
     if N is not None:
         cfg["N"] = int(N)
-    if chains is not None:
-        cfg["chains"] = int(chains)
-    g = torch.Generator(device=device)
-    g.manual_seed(cfg["seed"])
     N, D = cfg["N"], cfg["D"]
+    rng = np.random.default_rng(cfg["seed"])
     if name == "c1":
-        X = torch.randn((N, D - 1), generator=g, device=device)
-        X = torch.cat([X, torch.ones((N, 1), device=device)], 1)
-        beta = torch.randn(D, generator=g, device=device)
+        X = np.concatenate([_normal_rows(rng, N, D - 1), np.ones((N, 1), np.float32)], 1)
+        beta = rng.standard_normal(D)
     elif name == "c2":
-        Xc = torch.randn((N, 10), generator=g, device=device)
-        Xb = (torch.rand((N, 44), generator=g, device=device) < 0.1).float()
-        X = torch.cat([Xc, Xb, torch.ones((N, 1), device=device)], 1)
-        beta = 0.5 * torch.randn(D, generator=g, device=device)
+        Xc = _normal_rows(rng, N, 10)
+        Xb = (rng.uniform(size=(N, 44)) < 0.1).astype(np.float32)
+        X = np.concatenate([Xc, Xb, np.ones((N, 1), np.float32)], 1)
+        beta = 0.5 * rng.standard_normal(D)
     elif name == "c5":
-        X = torch.randn((N, D), generator=g, device=device)
-        cols = torch.randperm(D, generator=g, device=device)[:51]
+        X = _normal_rows(rng, N, D)
+        cols = rng.choice(D, 51, replace=False)
         X[:, cols] *= 10.0
-        beta = 0.1 * torch.randn(D, generator=g, device=device)
+        beta = 0.1 * rng.standard_normal(D)
     else:
         raise ValueError(name)
-    logits = X @ beta
-    y = (torch.rand(N, generator=g, device=device) < torch.sigmoid(logits)).float()
-    return X.contiguous(), y.contiguous(), beta, cfg
+    y = _labels(rng, X, beta)
+    return X, y, beta, cfg
+
+
+def make(name, device="cuda", N=None, chains=None):
+    """``design`` moved to `device` as torch tensors (beta* float32)."""
+    X, y, beta, cfg = design(name, N)
+    if chains is not None:
+        cfg["chains"] = int(chains)
+    return (torch.from_numpy(X).to(device), torch.from_numpy(y).to(device),
+            torch.from_numpy(beta.astype(np.float32)).to(device), cfg)
 
 
 def make_sv(T=3000, seed=3):
     """Returns generated from the stochastic-volatility model itself:
     sigma* ~ Exponential(50), nu* ~ Exponential(0.1), s_0 ~ N(0, sigma*),
     s_t = s_{t-1} + sigma* z_t, r_t = exp(s_t) * StudentT(nu*).  numpy PCG64."""
-    import numpy as np
     rng = np.random.default_rng(seed)
     sigma = rng.exponential(scale=1 / 50.0)
     nu = rng.exponential(scale=1 / 0.1)
@@ -67,7 +92,6 @@ def make_hier(G=1024, seed=4, nmin=50, nmax=5000, dmin=4, dmax=64):
     N_i ~ U{nmin..nmax}, D_i ~ U{dmin..dmax}, X_i ~ N(0,1), mu_i ~ N(0,1),
     tau_i ~ HalfCauchy(1), eta_i ~ N(0,1)^D_i, beta_i = mu_i + tau_i eta_i,
     y_i ~ Bernoulli(sigmoid(X_i beta_i)).  numpy PCG64."""
-    import numpy as np
     rng = np.random.default_rng(seed)
     Xs, ys = [], []
     for _ in range(G):
@@ -113,7 +137,6 @@ def laplace_init(X, y, C, seed=0, full_cov=False, return_mode=False):
     the initial inverse_mass_matrix (Laplace variances, or the Laplace
     covariance with ``full_cov``), both float32; with ``return_mode`` also
     the float64 mode and standard deviations."""
-    import numpy as np
     mode, cov = laplace(X, y, full_cov=full_cov)
     var = torch.diagonal(cov) if full_cov else cov
     sd = var.sqrt()
