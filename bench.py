@@ -48,11 +48,12 @@ DEFAULT_ITERS = {"c5": (120, 150), "c2": (1000, 1000), "c1": (500, 500), "c3": (
 
 
 def run_form(model, C, num_warmup, steps, z0, keys, state=None, collect=True, imm=None,
-             heuristic=False, dense=False, adapt_mass=True, launch_shape=None):
+             heuristic=False, dense=False, adapt_mass=True, regularize=True, launch_shape=None):
     import torch
     from numpyro_amd.infer.hmc import Engine
     eng = Engine(model, C, "NUTS", num_warmup, find_heuristic_step_size=heuristic,
-                 dense_mass=dense, adapt_mass_matrix=adapt_mass)
+                 dense_mass=dense, adapt_mass_matrix=adapt_mass,
+                 regularize_mass_matrix=regularize)
     if launch_shape:
         eng.set_launch_shape(launch_shape)
     eng.init(z0, keys, imm)
@@ -182,10 +183,11 @@ def main():
     ap.add_argument("--dense-mass", action="store_true",
                     help="logistic workloads: dense_mass=True, initial inverse_mass_matrix "
                          "the Laplace covariance")
-    ap.add_argument("--mass", default="adapt", choices=["adapt", "laplace"],
+    ap.add_argument("--mass", default="adapt", choices=["adapt", "adapt-unregularized", "laplace"],
                     help="logistic workloads: adapt the inverse mass matrix by Welford "
-                         "windows starting from the Laplace variances (default), or keep "
-                         "the Laplace matrix fixed (adapt_mass_matrix=False)")
+                         "windows starting from the Laplace variances (default), the same "
+                         "with regularize_mass_matrix=False, or keep the Laplace matrix "
+                         "fixed (adapt_mass_matrix=False)")
     args = ap.parse_args()
     steps, warm = DEFAULT_ITERS[args.config]
     args.steps = steps if args.steps is None else args.steps
@@ -228,10 +230,11 @@ def main():
         X, y, x_dtype=args.x_dtype,
         kernel=args.bf16_kernel if args.x_dtype == "bf16" else "auto")
     nw = max(args.warmup, MIN_ADAPT)
-    adapt_mass = args.mass == "adapt"
+    adapt_mass = args.mass != "laplace"
+    regularize = args.mass != "adapt-unregularized"
     eng, out, wall, warm_s, kms, kn = run_form(model, C, nw, args.steps, z0, keys,
                                                imm=imm, heuristic=True, dense=args.dense_mass,
-                                               adapt_mass=adapt_mass,
+                                               adapt_mass=adapt_mass, regularize=regularize,
                                                launch_shape=args.launch_shape)
     ge = eng.grad_evals
     ns = out["num_steps"].cpu().numpy()
@@ -271,7 +274,7 @@ def main():
         m32 = LogisticRegression(X, y, x_dtype="fp32", kernel="mfma")
         e32, o32, w32, _, k32, n32 = run_form(m32, C, nw, args.fp32_steps, z0, keys,
                                               state=post_state, dense=args.dense_mass,
-                                              adapt_mass=adapt_mass)
+                                              adapt_mass=adapt_mass, regularize=regularize)
         me32, rh32 = min_ess(o32["z"]) if args.fp32_steps >= 4 else (float("nan"), float("nan"))
         # the bf16 chains' ESS over the same number of draws, for comparison
         mes_b, _ = min_ess(out["z"][:, :args.fp32_steps])
@@ -328,6 +331,7 @@ def main():
                    "inverse_mass_matrix": ("Laplace covariance (dense_mass=True)"
                                            if args.dense_mass else "Laplace variances (diagonal)"),
                    "adapt_mass_matrix": adapt_mass,
+                   "regularize_mass_matrix": regularize,
                    "find_heuristic_step_size": True},
         "valid": valid,
         "setup_s": setup_s,

@@ -60,15 +60,16 @@ def _run_oracle(X, y, z0, keys, num_warmup, num_samples, algo="NUTS", **kw):
     return [np.stack(res[k]) for k in ("z", "num_steps", "potential_energy", "accept_prob")]
 
 
-@pytest.mark.parametrize("seed, adapt", [(11, True), (12, False)])
-def test_nuts_chain_matches_oracle(seed, adapt):
+@pytest.mark.parametrize("seed, adapt, regularize", [(11, True, True), (12, False, True),
+                                                     (13, True, False)])
+def test_nuts_chain_matches_oracle(seed, adapt, regularize):
     N, D, C = 200, 5, 3
     X, y = _problem(N, D, seed)
     rng = np.random.default_rng(seed + 100)
     z0 = rng.uniform(-2, 2, size=(C, D)).astype(np.float32)
     keys = np.stack([ojr.split(ojr.PRNGKey(seed), C)[c] for c in range(C)])
     kw = dict(adapt_step_size=adapt, adapt_mass_matrix=adapt,
-              step_size=1.0 if adapt else 0.2)
+              step_size=1.0 if adapt else 0.2, regularize_mass_matrix=regularize)
     num_warmup, num_samples = 40, 20
     gz, gns, gpe, gap = _run_engine(X, y, z0, keys, num_warmup, num_samples, **kw)
     oz, ons, ope, oap = _run_oracle(X, y, z0, keys, num_warmup, num_samples, **kw)
