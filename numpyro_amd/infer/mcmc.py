@@ -201,6 +201,17 @@ class MCMC:
         return {k: (v if group_by_chain else self._flat(v))
                 for k, v in self._states.items() if k != self._sample_field}
 
+    def transfer_states_to_host(self):
+        """``mcmc.py:799-804``: moves the collected samples and extra fields
+        from HBM to host memory (torch CPU tensors)."""
+        def host(x):
+            if isinstance(x, dict):
+                return {k: host(v) for k, v in x.items()}
+            return x.cpu() if torch.is_tensor(x) else x
+        self._states = host(self._states)
+        if self._states_flat is not None:
+            self._states_flat = host(self._states_flat)
+
     def print_summary(self, prob=0.9, exclude_deterministic=True):
         """``mcmc.py:769-797``."""
         sites = self._states[self._sample_field]

@@ -84,6 +84,12 @@ def test_extra_fields_shapes():
     assert st["num_steps"].shape == (1, 300)
     assert st["adapt_state.step_size"].shape == (1, 300)
     assert (st["num_steps"] >= 1).all()
+    # mcmc.py:799-804: the same samples, now in host memory
+    before = mcmc.get_samples().cpu()
+    mcmc.transfer_states_to_host()
+    after = mcmc.get_samples()
+    assert after.device.type == "cpu" and torch.equal(before, after)
+    assert mcmc.get_extra_fields()["num_steps"].device.type == "cpu"
 
 
 def test_post_warmup_state_resume():
