@@ -253,16 +253,17 @@ __device__ double hier_group_eval(const HierData& hd, int i, int sub, const floa
   return U;
 }
 
-template <int K>
-__global__ void __launch_bounds__(64 * K)
+template <int K, int R>
+__global__ void __launch_bounds__(64 * K * R)
 hier_eval_kernel(HierData hd, const float* __restrict__ zreq, int ldz, double* __restrict__ pe,
                  float* __restrict__ gout, int ldg) {
-  __shared__ HierShared<K, 1> sh;
+  __shared__ HierShared<K, R> sh;
   const int per = hd.K / K;  // workgroups per model
   const int i = hd.order[blockIdx.x / per];
   const int sub = blockIdx.x % per;
-  const double u = hier_group_eval<K, 1>(hd, i, sub, zreq, ldz, gout, ldg, sh);
-  if ((threadIdx.x & 63) == 0) pe[i * hd.K + sub * K + (threadIdx.x >> 6)] = u;
+  const double u = hier_group_eval<K, R>(hd, i, sub, zreq, ldz, gout, ldg, sh);
+  const int wa = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0 && wa % R == 0) pe[i * hd.K + sub * K + wa / R] = u;
 }
 
 // One block per (model, chain subset): K chain slots of R waves each.  Wave
@@ -390,15 +391,25 @@ struct HierPotential : Potential {
   hipError_t eval(hipStream_t s, const float* z, int ldz, int Cn, double* pe, float* g,
                   int ldg) override {
     if (Cn != C || ldz < maxd + 2 || ldg < maxd + 2) return hipErrorInvalidValue;
-    const int w = waves();
+    // one evaluation of every chain: one wave per chain unless a shape was
+    // fixed (C4 at fixed positions: 0.224 ms with one wave per chain, 0.299
+    // with two, 0.255 with four -- the waves per chain pay off only on the
+    // heavy models' chains that the persistent launch waits for)
+    const int w = waves(), r = forced ? rw : 1;
     const dim3 grid(groups * (hd.K / w));
+    bool launched = false;
     mark_begin(s);
-    switch (w) {
-#define NAX_K(KV) case KV: hipLaunchKernelGGL(hier_eval_kernel<KV>, grid, dim3(64 * KV), 0, s, hd, z, ldz, pe, g, ldg); break;
-      NAX_K(1) NAX_K(2) NAX_K(4) NAX_K(8) NAX_K(16)
-#undef NAX_K
-      default: return hipErrorInvalidValue;
-    }
+#define NAX_KR(KV, RV)                                                                        \
+  if (!launched && w == KV && r == RV) {                                                      \
+    hipLaunchKernelGGL((hier_eval_kernel<KV, RV>), grid, dim3(64 * KV * RV), 0, s, hd, z,     \
+                       ldz, pe, g, ldg);                                                      \
+    launched = true;                                                                          \
+  }
+    NAX_KR(1, 1) NAX_KR(2, 1) NAX_KR(4, 1) NAX_KR(8, 1) NAX_KR(16, 1)
+    NAX_KR(1, 2) NAX_KR(2, 2) NAX_KR(4, 2)
+    NAX_KR(1, 4) NAX_KR(2, 4) NAX_KR(4, 4)
+#undef NAX_KR
+    if (!launched) return hipErrorInvalidValue;
     mark_end(s);
     return hipGetLastError();
   }

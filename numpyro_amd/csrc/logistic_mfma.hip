@@ -214,23 +214,34 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   CompSum u;  // fp64 energy over this lane's tiles, compensated
 
   // ---- tile staging: global -> registers (prefetch) -> LDS ----
-  uint4 pf[PF];
-  float4 pfy = make_float4(0.f, 0.f, 0.f, 0.f);
+  // Clang vector types, not HIP's uint4/float4 structs: an array of those
+  // stays in scratch memory, and every prefetch then waits for its loads.
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  u32x4 pf[PF];
+  f32x4 pfy;
   auto load_tile = [&](int64_t t) {
-    const uint4* ga = (const uint4*)(A1 + (size_t)t * KK * 64);
-    const uint4* gb = (const uint4*)(B2 + (size_t)t * M2 * DT * 64);
+    const u32x4* ga = (const u32x4*)(A1 + (size_t)t * KK * 64);
+    const u32x4* gb = (const u32x4*)(B2 + (size_t)t * M2 * DT * 64);
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
       const int idx = tid + 256 * p;
-      pf[p] = idx < PA ? ga[idx] : gb[idx - PA];
+      if constexpr (PA % 256 == 0) {  // the source of chunk p is known at compile time
+        if (p < PA / 256) pf[p] = ga[idx];
+        else pf[p] = gb[idx - PA];
+      } else {
+        pf[p] = idx < PA ? ga[idx] : gb[idx - PA];
+      }
     }
-    if (tid < 8) pfy = ((const float4*)(ypad + 32 * t))[tid];
+    pfy = ((const f32x4*)(ypad + 32 * t))[tid & 7];
   };
   auto store_tile = [&](int buf) {
-    uint4* s4 = (uint4*)lds[buf];
+    u32x4* s4 = (u32x4*)lds[buf];
 #pragma unroll
     for (int p = 0; p < PF; ++p) s4[tid + 256 * p] = pf[p];
-    if (tid < 8) ((float4*)(lds[buf] + ABYTES + BBYTES))[tid] = pfy;
+    // every thread stores its copy of label chunk tid & 7 (equal values):
+    // an unconditional store keeps the label load at the top of the tile
+    ((f32x4*)(lds[buf] + ABYTES + BBYTES))[tid & 7] = pfy;
   };
 
   int64_t t = rb;
@@ -242,7 +253,12 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
   __syncthreads();
   for (; t < ntile; t += RB) {
     const bool has_next = t + RB < ntile;
-    if (has_next) load_tile(t + RB);
+    // unconditional (the last tile reloads itself, unused), so that the
+    // prefetch registers are written on every path and stay registers
+    load_tile(has_next ? t + RB : t);
+    // keep the prefetch loads here: left to itself the scheduler sinks them
+    // next to their LDS stores at the end of the tile and waits on them
+    __builtin_amdgcn_sched_barrier(0);
     const frag* la = (const frag*)lds[cur];
     const frag* lb = (const frag*)(lds[cur] + ABYTES);
     const float* ly = (const float*)(lds[cur] + ABYTES + BBYTES);
@@ -301,7 +317,7 @@ logit_mfma(const typename F::frag* __restrict__ A1, const typename F::frag* __re
         }
       }
     }
-    if (has_next) store_tile(cur ^ 1);
+    store_tile(cur ^ 1);  // unconditional: the other buffer is not read after the last tile
     __syncthreads();
     cur ^= 1;
   }

@@ -28,6 +28,8 @@ def main(argv=None):
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--launch-shape", type=int, default=None,
+                    help="c4: chains per workgroup + 32 x (waves per chain - 1)")
     args = ap.parse_args(argv)
 
     from . import _lib, workloads
@@ -80,6 +82,8 @@ def main(argv=None):
             model = LogisticRegression(X, y, x_dtype=args.x_dtype, kernel=args.kernel)
             z = (beta[None, :] + 1e-3 * torch.randn((C_, D), device="cuda")).contiguous()
         eng = Engine(model, C_, "NUTS", 0)
+        if args.launch_shape:
+            eng.set_launch_shape(args.launch_shape)
         pe = torch.empty(C_, dtype=torch.float64, device="cuda")
         g = torch.empty((C_, D), dtype=torch.float32, device="cuda")
         lib = _lib.load()

@@ -26,8 +26,11 @@ def _groups(shapes, seed):
 SHAPES = [(1, 1), (63, 4), (64, 17), (65, 64), (200, 9), (1000, 33), (5000, 64), (129, 63)]
 
 
-@pytest.mark.parametrize("K", [1, 4])
-def test_hier_potential_grad(K):
+@pytest.mark.parametrize("K, shape", [(1, None), (4, None), (4, 36), (4, 100), (4, 34),
+                                      (1, 97)])
+def test_hier_potential_grad(K, shape):
+    # shape: chains per workgroup + 32 (waves per chain - 1), e.g. 100 = four
+    # chains of four waves, each wave on 64 of a 256-row tile
     from numpyro_amd import _lib
     from numpyro_amd.infer.hmc import Engine
     from numpyro_amd.models import HierarchicalLogistic
@@ -35,6 +38,8 @@ def test_hier_potential_grad(K):
     m = HierarchicalLogistic(Xs, ys, chains_per_group=K)
     C = len(SHAPES) * K
     eng = Engine(m, C, "NUTS", 0)
+    if shape:
+        eng.set_launch_shape(shape)
     rng = np.random.default_rng(5)
     Z = np.zeros((C, m.dim), np.float32)
     for c in range(C):
