@@ -199,7 +199,15 @@ logit_partial_f32(XL X, const float* __restrict__ y, int64_t N, int D,
   }
 }
 
-// Sum partials in rb order, add the Normal(0, s) prior.
+// Sums the row-split partials and adds the Normal(0, s) prior; block ci
+// reduces workspace column ci.  Gradient: with D <= 128 the 256 threads form
+// 256 / DP2 groups (DP2 = D rounded up to a power of two), thread (group q,
+// column d) adds the splits q, q + groups, ... in order and the group sums
+// are added in group order; wider rows take a column per thread.  Energy:
+// thread t adds the splits t, t + 256, ... into a compensated sum and the
+// 256 sums are combined in a fixed pairwise tree.  Every order is fixed, so
+// results repeat bit for bit; the splits are read by all threads at once
+// instead of one column's (or, for the energy, one thread's) dependent loop.
 __global__ void __launch_bounds__(256)
 logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int RB, int C,
              int D, const float* __restrict__ Z, int ldz, double inv_s2, double log_norm,
@@ -222,26 +230,61 @@ logit_reduce(const float* __restrict__ wsg, const double* __restrict__ wsu, int 
   }
   const int t = threadIdx.x;
   __shared__ double red[256];
+  __shared__ double us[256], uc[256];
+  __shared__ float gred[256];
+  int dp2 = 1;
+  while (dp2 < D) dp2 <<= 1;
   double pr = 0.0;
-  for (int d = t; d < D; d += 256) {
-    float s = 0.0f;
-    for (int r = 0; r < RB; ++r) s += wsg[((size_t)r * ldw + ci) * D + d];
-    const float zv = Z[(size_t)c * ldz + d];
-    g[(size_t)c * ldg + d] = s + (float)(zv * inv_s2);
-    pr += 0.5 * (double)zv * zv * inv_s2 + log_norm;
+  if (dp2 <= 128) {
+    const int ng = 256 / dp2, q = t / dp2, d = t % dp2;
+    float sg = 0.0f;
+    if (d < D)
+      for (int r = q; r < RB; r += ng) sg += wsg[((size_t)r * ldw + ci) * D + d];
+    gred[t] = sg;
+    __syncthreads();
+    if (q == 0 && d < D) {
+      float tot = 0.0f;
+      for (int k = 0; k < ng; ++k) tot += gred[k * dp2 + d];
+      const float zv = Z[(size_t)c * ldz + d];
+      g[(size_t)c * ldg + d] = tot + (float)(zv * inv_s2);
+      pr = 0.5 * (double)zv * zv * inv_s2 + log_norm;
+    }
+  } else {
+    for (int d = t; d < D; d += 256) {
+      float sg = 0.0f;
+      for (int r = 0; r < RB; ++r) sg += wsg[((size_t)r * ldw + ci) * D + d];
+      const float zv = Z[(size_t)c * ldz + d];
+      g[(size_t)c * ldg + d] = sg + (float)(zv * inv_s2);
+      pr += 0.5 * (double)zv * zv * inv_s2 + log_norm;
+    }
   }
+  CompSum u;  // this thread's row-split energy partials, compensated
+  for (int r = t; r < RB; r += 256) u.add(wsu[(size_t)r * ldw + ci]);
   red[t] = pr;
+  us[t] = u.s;
+  uc[t] = u.c;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) red[t] += red[t + o];
+    if (t < o) {
+      red[t] += red[t + o];
+      CompSum a, b;
+      a.s = us[t];
+      a.c = uc[t];
+      b.s = us[t + o];
+      b.c = uc[t + o];
+      a.add(b);
+      us[t] = a.s;
+      uc[t] = a.c;
+    }
     __syncthreads();
   }
   if (t == 0) {
-    CompSum u;  // the RB row-split partials, compensated
-    for (int r = 0; r < RB; ++r) u.add(wsu[(size_t)r * ldw + ci]);
-    u.add(red[0]);
-    u.add(u_offset);
-    pe[c] = u.value();
+    CompSum tot;
+    tot.s = us[0];
+    tot.c = uc[0];
+    tot.add(red[0]);
+    tot.add(u_offset);
+    pe[c] = tot.value();
   }
 }
 
