@@ -269,8 +269,9 @@ int nax_tuning(nax_sampler* h, char* buf, int32_t len);
  * per workgroup kb (a divisor of chains_per_group) and waves per chain
  * r (1, 2 or 4; kb r <= 16) as shape = kb + 32 (r - 1).  Draws depend on
  * the waves per chain (the order of its sums), not on the chains per
- * workgroup; a fixed shape makes runs repeat bit for bit.  No counterpart
- * in the reference. */
+ * workgroup; a fixed shape makes runs repeat bit for bit.  A fixed shape
+ * also sets the waves per chain of nax_potential_grad for these models
+ * (one wave per chain otherwise).  No counterpart in the reference. */
 int nax_set_launch_shape(nax_sampler* h, int32_t shape);
 
 /* HMCState / post_warmup_state access (mcmc.py:558-587): copies the

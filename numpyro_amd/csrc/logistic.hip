@@ -469,7 +469,7 @@ Potential* make_logistic_mfma(const float* x, const float* y, int64_t n, int d,
                               std::string* err);
 
 // fuse_variant: 0 = auto (matrix cores when num_chains >= 32), 1 = VALU tile
-// kernel, 2 = matrix cores, 3 = one wavefront per chain.  bf16 always runs on
+// kernel, 2 = matrix cores, 3 = a group of wavefronts per chain.  bf16 always runs on
 // the matrix cores: 0 or 4 keep one X copy (bf16 residuals in X^T R), 2 keeps
 // a second, fp16 copy for X^T R with fp16 residuals.
 Potential* make_logistic(const float* x, const float* y, int64_t n, int d, double prior_scale,
@@ -494,8 +494,8 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
     *err = "x_dtype not supported";
     return nullptr;
   }
-  // a handful of chains on a small design: one wavefront per chain, the
-  // whole sampling loop in one persistent launch
+  // a handful of chains on a small design: a group of wavefronts per chain,
+  // the whole sampling loop in one persistent launch
   const bool fp8 = x_dtype == NAX_X_FP8;
   if (!fp8 && (fuse_variant == 3 || (fuse_variant == 0 && num_chains <= 16 && d <= 64 &&
                                      (double)n * d <= (double)(1 << 20))))

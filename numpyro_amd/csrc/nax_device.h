@@ -176,15 +176,67 @@ __device__ __forceinline__ bool bernoulli(Key k, double p, bool part) {
 }
 
 // ---- wave (64-lane) reductions ---------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NAX_WAVE);
-  return v;
+// Called with all 64 lanes active.  Lanes combine inside rows of 16 through
+// DPP lane permutations (VALU moves, no LDS round trip as a shuffle takes);
+// the row totals are read out of lanes 0/16/32/48 (or 15/31/47 for a scan)
+// and combined in lane order, so every lane returns the same value.
+namespace dpp {
+constexpr int kXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kHalfMirror = 0x141;  // lane i <-> 7 - i inside each 8 lanes
+constexpr int kMirror = 0x140;      // lane i <-> 15 - i inside each row
+constexpr int kShr1 = 0x111;        // row_shr: lane i reads i - n of its row (0 below)
+constexpr int kShr2 = 0x112;
+constexpr int kShr4 = 0x114;
+constexpr int kShr8 = 0x118;
+}  // namespace dpp
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NAX_WAVE);
-  return v;
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF,
+                                                            false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL,
+                                                            0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum_t(T v) {
+  v += dpp_mov<dpp::kXor1>(v);  // pairs
+  v += dpp_mov<dpp::kXor2>(v);  // quads
+  v += dpp_mov<dpp::kHalfMirror>(v);  // half rows
+  v += dpp_mov<dpp::kMirror>(v);  // rows
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_t(v); }
+__device__ __forceinline__ float wave_sum(float v) { return wave_sum_t(v); }
+
+// Inclusive prefix sum over the wave's lanes in lane order: a scan inside
+// each row of 16, then the totals of the rows before.
+__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
+  v += dpp_mov<dpp::kShr1>(v);
+  v += dpp_mov<dpp::kShr2>(v);
+  v += dpp_mov<dpp::kShr4>(v);
+  v += dpp_mov<dpp::kShr8>(v);
+  const double t0 = read_lane(v, 15), t1 = read_lane(v, 31), t2 = read_lane(v, 47);
+  const int row = lane >> 4;
+  const double off = row == 0 ? 0.0 : row == 1 ? t0 : row == 2 ? t0 + t1 : (t0 + t1) + t2;
+  return v + off;
 }
 
 // Work split of the MFMA logistic kernels for n_act requesting chains over a

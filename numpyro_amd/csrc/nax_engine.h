@@ -146,15 +146,6 @@ __device__ __forceinline__ double chain_sum(const G& g, double x) {
 }
 
 // Inclusive prefix sum over the group's threads in lane order.
-__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
-#pragma unroll
-  for (int o = 1; o < NAX_WAVE; o <<= 1) {
-    const double t = __shfl_up(v, o, NAX_WAVE);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
-
 template <class G>
 __device__ __forceinline__ double chain_inclusive_scan(const G& g, double v) {
   v = wave_inclusive_scan(v, g.lane & (NAX_WAVE - 1));
@@ -902,6 +893,33 @@ perchain_eval_group_kernel(M m, const float* __restrict__ z, int ldz, int C,
   grp.lane = threadIdx.x;
   const double u = m.value_and_grad(grp, z + (size_t)c * ldz, g + (size_t)c * ldg);
   if (grp.lane == 0) pe[c] = u;
+}
+
+// One-shot evaluation of C chains at wpc wavefronts per chain (kGroupSizes):
+// one wave per chain packs four chains into a workgroup, more waves give each
+// chain a workgroup of its own.
+template <class M>
+hipError_t launch_group_eval(int wpc, hipStream_t s, const M& m, const float* z, int ldz, int C,
+                             double* pe, float* g, int ldg) {
+  switch (wpc) {
+    case 1:
+      hipLaunchKernelGGL(perchain_eval_kernel<M>, dim3((C + 3) / 4), dim3(256), 0, s, m, z, ldz,
+                         C, pe, g, ldg);
+      break;
+#define NAX_GROUP_EVAL(W)                                                                  \
+  case W:                                                                                  \
+    hipLaunchKernelGGL((perchain_eval_group_kernel<M, W>), dim3(C), dim3(64 * W), 0, s, m, z, \
+                       ldz, C, pe, g, ldg);                                                \
+    break;
+      NAX_GROUP_EVAL(2)
+      NAX_GROUP_EVAL(4)
+      NAX_GROUP_EVAL(8)
+      NAX_GROUP_EVAL(16)
+#undef NAX_GROUP_EVAL
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 // Vector slots a persistent launch with one wavefront per workgroup keeps in

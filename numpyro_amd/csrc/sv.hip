@@ -208,11 +208,11 @@ struct SVPotential : Potential {
 
   hipError_t eval(hipStream_t s, const float* z, int ldz, int C, double* pe, float* g,
                   int ldg) override {
+    // one wave per chain unless a launch shape is fixed
     mark_begin(s);
-    hipLaunchKernelGGL(perchain_eval_kernel<SVModel>, dim3((C + 3) / 4), dim3(256), 0, s, m, z,
-                       ldz, C, pe, g, ldg);
+    const hipError_t e = launch_group_eval(forced ? wpc : 1, s, m, z, ldz, C, pe, g, ldg);
     mark_end(s);
-    return hipGetLastError();
+    return e;
   }
   hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
                                 double* wel, float* zreq, float* gout, double* peout) override {

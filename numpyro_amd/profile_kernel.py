@@ -29,7 +29,8 @@ def main(argv=None):
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--launch-shape", type=int, default=None,
-                    help="c4: chains per workgroup + 32 x (waves per chain - 1)")
+                    help="c4: chains per workgroup + 32 x (waves per chain - 1); "
+                         "c1, c3, c3nc: waves per chain")
     args = ap.parse_args(argv)
 
     from . import _lib, workloads

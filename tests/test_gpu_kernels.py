@@ -127,14 +127,20 @@ def test_logistic_potential_grad_fp32(lib, N, D, C):
         np.testing.assert_allclose(g[c].cpu().numpy(), gr, rtol=1e-4, atol=1e-5 * scale)
 
 
-@pytest.mark.parametrize("N, D, C", [(1000, 26, 4), (1, 3, 1), (777, 55, 9), (100, 1, 3),
-                                     (5000, 64, 2), (63, 33, 5)])
-def test_logistic_potential_grad_chain_kernel(lib, N, D, C):
-    # one wavefront per chain (kernel "chain"): lane = row, d <= 64
+@pytest.mark.parametrize("N, D, C, waves", [(1000, 26, 4, None), (1, 3, 1, None),
+                                            (777, 55, 9, None), (100, 1, 3, None),
+                                            (5000, 64, 2, None), (63, 33, 5, None),
+                                            (1000, 26, 4, 4), (777, 55, 9, 2), (5000, 64, 2, 16),
+                                            (63, 33, 5, 8), (1, 3, 1, 16)])
+def test_logistic_potential_grad_chain_kernel(lib, N, D, C, waves):
+    # a group of waves per chain (kernel "chain"): rows split over the
+    # group's threads, d <= 64; waves fixed by the launch shape, else one
     from numpyro_amd.infer.hmc import Engine
     from numpyro_amd.models import LogisticRegression
     X, y, Z = _logistic_problem(N, D, C, N + D + C + 1)
     eng = Engine(LogisticRegression(X, y, prior_scale=0.9, kernel="chain"), C, "NUTS", 0)
+    if waves:
+        eng.set_launch_shape(waves)
     z = torch.from_numpy(Z).cuda()
     pe = torch.empty(C, dtype=torch.float64, device="cuda")
     g = torch.empty((C, D), dtype=torch.float32, device="cuda")

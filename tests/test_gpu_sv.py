@@ -33,14 +33,20 @@ def _points(T, C, noncentred, seed):
 
 
 @pytest.mark.parametrize("noncentred", [False, True])
-@pytest.mark.parametrize("T, C", [(3000, 16), (5, 3), (64, 4), (65, 5), (130, 8), (4097, 2)])
-def test_sv_potential_grad(noncentred, T, C):
+@pytest.mark.parametrize("T, C, waves", [(3000, 16, None), (5, 3, None), (64, 4, None),
+                                         (65, 5, None), (130, 8, None), (4097, 2, None),
+                                         (3000, 16, 4), (3000, 16, 16), (5, 3, 2),
+                                         (1025, 5, 16), (4097, 2, 8)])
+def test_sv_potential_grad(noncentred, T, C, waves):
+    # waves per chain fixed by the launch shape, else one
     from numpyro_amd import _lib
     from numpyro_amd.infer.hmc import Engine
     from numpyro_amd.models import StochasticVolatility
     r = _returns(T, 30 + T)
     Z = _points(T, C, noncentred, T + C)
     eng = Engine(StochasticVolatility(r, noncentred=noncentred), C, "NUTS", 0)
+    if waves:
+        eng.set_launch_shape(waves)
     z = torch.from_numpy(Z).cuda()
     pe = torch.empty(C, dtype=torch.float64, device="cuda")
     g = torch.empty((C, T + 2), dtype=torch.float32, device="cuda")
