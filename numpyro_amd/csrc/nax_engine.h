@@ -254,7 +254,8 @@ __device__ __forceinline__ void draw_momentum(const CX& cx, int dst, Key k, bool
 }
 
 // First half of velocity_verlet (hmc_util.py:289-297) on an edge, in place:
-// r <- r - eps/2 g; z <- z + eps M^-1 r; z is also written to the request.
+// r <- r - eps/2 g; z <- z + eps M^-1 r; z is also written to the request
+// (unless zreq is null: the potential then reads the edge's slot).
 template <class CX>
 __device__ __forceinline__ void leapfrog_half(const CX& cx, int zs, int rs, int gs, float eps,
                               float* zreq) {
@@ -269,7 +270,7 @@ __device__ __forceinline__ void leapfrog_half(const CX& cx, int zs, int rs, int 
     FOR_D(d) {
       const float zn = z[d] + eps * dense_mv(cx, cx.mi, r, d);
       z[d] = zn;
-      zreq[d] = zn;
+      if (zreq) zreq[d] = zn;
     }
     return;
   }
@@ -278,7 +279,7 @@ __device__ __forceinline__ void leapfrog_half(const CX& cx, int zs, int rs, int 
     float zn = z[d] + eps * (m[d] * rh);
     r[d] = rh;
     z[d] = zn;
-    zreq[d] = zn;
+    if (zreq) zreq[d] = zn;
   }
 }
 
@@ -294,6 +295,65 @@ __device__ __forceinline__ void leapfrog_finish(const CX& cx, int rs, int gs, fl
     g[d] = gv;
     r[d] = r[d] - h * gv;
   }
+}
+
+// A leaf's vector work in one pass (diagonal mass): the second half of
+// velocity_verlet (g <- gin, r <- r - eps/2 g), the kinetic energy's sum
+// r . M^-1 r, the subtree's r_sum (set at its first leaf, else += r), at an
+// even leaf the checkpoints (r, r_sum) into ckr_w / cks_w, and at an odd leaf
+// _is_turning's two sums for each of the NCHK checkpoints ck[j] (r_sum taken
+// from checkpoint j on, hmc_util.py:961-981); one reduction for all of them.
+// Each value is the same expression summed in the same order as by
+// leapfrog_finish, kinetic and is_turning one after the other.
+template <int NCHK, class CX>
+__device__ __forceinline__ void leaf_pass(const CX& cx, int rs, int gs, float eps,
+                                          const float* gin, bool first, float* ckr_w,
+                                          float* cks_w, const float* ckr0, const float* cks0,
+                                          const float* ckr1, const float* cks1,
+                                          const float* ckr2, const float* cks2,
+                                          double (&out)[1 + 2 * NCHK]) {
+  const float* ckr[3] = {ckr0, ckr1, ckr2};
+  const float* cks[3] = {cks0, cks1, cks2};
+  float* r = cx.v(rs);
+  float* g = cx.v(gs);
+  float* srs = cx.v(V_SRSUM);
+  const float* m = cx.v(V_IMM);
+  const float h = 0.5f * eps;
+  double acc = 0.0;
+  double a[NCHK > 0 ? NCHK : 1], b[NCHK > 0 ? NCHK : 1];
+#pragma unroll
+  for (int j = 0; j < NCHK; ++j) a[j] = b[j] = 0.0;
+  FOR_D(d) {
+    const float gv = gin[d];
+    g[d] = gv;
+    const float rv = r[d] - h * gv;
+    r[d] = rv;
+    const double rd = rv;
+    acc += (double)m[d] * rd * rd;
+    const float s = first ? rv : srs[d] + rv;
+    srs[d] = s;
+    if constexpr (NCHK == 0) {
+      ckr_w[d] = rv;
+      cks_w[d] = s;
+    }
+#pragma unroll
+    for (int j = 0; j < NCHK; ++j) {
+      const float rlv = ckr[j][d];
+      const float sj = s - cks[j][d] + rlv;
+      const float c = sj - (rlv + rv) * 0.5f;
+      const float vl = m[d] * rlv;
+      const float vr = m[d] * rv;
+      a[j] += (double)vl * c;
+      b[j] += (double)vr * c;
+    }
+  }
+  out[0] = acc;
+#pragma unroll
+  for (int j = 0; j < NCHK; ++j) {
+    out[1 + 2 * j] = a[j];
+    out[2 + 2 * j] = b[j];
+  }
+  chain_sum_n(cx, out);
 }
 
 // ---- warmup adaptation (hmc_util.py:60-130, 133-239, 518-707) ---------------
@@ -667,26 +727,95 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
         const int rs = S.going_right ? V_RR : V_RL;
         const int gs = S.going_right ? V_GR : V_GL;
         const float eps = (float)(S.going_right ? S.step_size : -S.step_size);
-        leapfrog_finish(cx, rs, gs, eps, gi);
+        const int leaf_idx = S.s_num;
+        // _leaf_idx_to_ckpt_idxs (hmc_util.py:941-958)
+        const int idx_max = __popc((unsigned)leaf_idx >> 1);
+        const int nsub = __popc((unsigned)((~leaf_idx & (leaf_idx + 1)) - 1));
+        const int idx_min = idx_max - nsub + 1;
+        const bool even = (leaf_idx & 1) == 0;  // even: writes checkpoint idx_max, no checks
+        const float* znew = cx.v(zs);
+        const float* rnew = cx.v(rs);
+        const float* gnew = cx.v(gs);
+        float* srs = cx.v(V_SRSUM);
+        double kin;
+        bool turning = false;
+        int next_check = idx_max;  // U-turn checks still to make: next_check down to idx_min
+        if (cx.mi) {
+          leapfrog_finish(cx, rs, gs, eps, gi);
+          kin = kinetic(cx, rs);
+          FOR_D(d) srs[d] = leaf_idx == 0 ? rnew[d] : srs[d] + rnew[d];
+          if (even) {
+            float* ckr = cx.v(CKR + idx_max);
+            float* cks = cx.v(CKS + idx_max);
+            FOR_D(d) { ckr[d] = rnew[d]; cks[d] = srs[d]; }
+          }
+        } else {  // the same values from one pass and one reduction
+          const int nf = even ? 0 : (nsub < 3 ? nsub : 3);
+          // checkpoints idx_max, idx_max - 1, idx_max - 2 (those beyond nf unused)
+          const int i1 = nf > 1 ? idx_max - 1 : idx_max, i2 = nf > 2 ? idx_max - 2 : idx_max;
+          const float* r0 = cx.v(CKR + idx_max);
+          const float* s0 = cx.v(CKS + idx_max);
+          const float* r1 = cx.v(CKR + i1);
+          const float* s1 = cx.v(CKS + i1);
+          const float* r2 = cx.v(CKR + i2);
+          const float* s2 = cx.v(CKS + i2);
+          float* ckr_w = even ? cx.v(CKR + idx_max) : nullptr;
+          float* cks_w = even ? cx.v(CKS + idx_max) : nullptr;
+          const bool first = leaf_idx == 0;
+          // _is_turning's verdict for checkpoint j of the pass: either sum <= 0
+          auto turn = [](const double* o, int j) {
+            return o[1 + 2 * j] <= 0.0 || o[2 + 2 * j] <= 0.0;
+          };
+          switch (nf) {
+            case 0: {
+              double o[1];
+              leaf_pass<0>(cx, rs, gs, eps, gi, first, ckr_w, cks_w, r0, s0, r1, s1, r2, s2,
+                           o);
+              kin = 0.5 * o[0];
+              break;
+            }
+            case 1: {
+              double o[3];
+              leaf_pass<1>(cx, rs, gs, eps, gi, first, ckr_w, cks_w, r0, s0, r1, s1, r2, s2,
+                           o);
+              kin = 0.5 * o[0];
+              turning = turn(o, 0);
+              break;
+            }
+            case 2: {
+              double o[5];
+              leaf_pass<2>(cx, rs, gs, eps, gi, first, ckr_w, cks_w, r0, s0, r1, s1, r2, s2,
+                           o);
+              kin = 0.5 * o[0];
+              turning = turn(o, 0) || turn(o, 1);
+              break;
+            }
+            default: {
+              double o[7];
+              leaf_pass<3>(cx, rs, gs, eps, gi, first, ckr_w, cks_w, r0, s0, r1, s1, r2, s2,
+                           o);
+              kin = 0.5 * o[0];
+              turning = turn(o, 0) || turn(o, 1) || turn(o, 2);
+              break;
+            }
+          }
+          next_check = idx_max - nf;
+        }
+        for (int i = next_check; i >= idx_min && !turning; --i)
+          turning = is_turning(cx, cx.v(CKR + i), rnew, srs, cx.v(CKS + i), cx.v(CKR + i));
         const double pe_new = pe_in;
-        const double e_new = pe_new + kinetic(cx, rs);
+        const double e_new = pe_new + kin;
         const double delta = nan_to_inf(e_new - S.energy_current);
         const double leaf_w = -delta;
         const int leaf_div = delta > cfg.max_delta_energy;
         const double acc = clip_exp_neg(delta);
-        const int leaf_idx = S.s_num;
-        const float* znew = cx.v(zs);
-        const float* rnew = cx.v(rs);
-        const float* gnew = cx.v(gs);
         float* szp = cx.v(V_SZP);
         float* sgp = cx.v(V_SGP);
-        float* srs = cx.v(V_SRSUM);
         if (leaf_idx == 0) {
-          FOR_D(d) { szp[d] = znew[d]; sgp[d] = gnew[d]; srs[d] = rnew[d]; }
+          FOR_D(d) { szp[d] = znew[d]; sgp[d] = gnew[d]; }
           S.s_peP = pe_new; S.s_energyP = e_new; S.s_weight = leaf_w;
           S.s_diverging = leaf_div; S.s_sum_accept = acc; S.s_num = 1;
         } else {  // _combine_tree(..., biased_transition=False) (hmc_util.py:767-848)
-          FOR_D(d) srs[d] += rnew[d];
           const double p = expit(leaf_w - S.s_weight);
           if (bernoulli(K(S.leaf_key), p, part)) {
             FOR_D(d) { szp[d] = znew[d]; sgp[d] = gnew[d]; }
@@ -697,18 +826,6 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
           S.s_sum_accept += acc;
           S.s_num += 1;
         }
-        // _leaf_idx_to_ckpt_idxs (hmc_util.py:941-958)
-        const int idx_max = __popc((unsigned)leaf_idx >> 1);
-        const int nsub = __popc((unsigned)((~leaf_idx & (leaf_idx + 1)) - 1));
-        const int idx_min = idx_max - nsub + 1;
-        if ((leaf_idx & 1) == 0) {
-          float* ckr = cx.v(CKR + idx_max);
-          float* cks = cx.v(CKS + idx_max);
-          FOR_D(d) { ckr[d] = rnew[d]; cks[d] = srs[d]; }
-        }
-        bool turning = false;
-        for (int i = idx_max; i >= idx_min && !turning; --i)
-          turning = is_turning(cx, cx.v(CKR + i), rnew, srs, cx.v(CKS + i), cx.v(CKR + i));
         S.s_turning = turning ? 1 : 0;
         if (S.s_num < (1 << S.depth) && !turning && !S.s_diverging) S.phase = PH_LEAF;
         else S.phase = PH_COMBINE;
@@ -807,6 +924,15 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
     }
   }
   return need_grad;
+}
+
+// Vector slots of the position a chain waiting for a gradient asked about and
+// of the gradient its waiting phase takes: the leaf's edge, or the right edge
+// for the step-size search and fixed-length HMC.
+__device__ __forceinline__ void request_slots(const ChainState& S, int& zs, int& gs) {
+  const bool left = S.phase == PH_LEAF_WAIT && !S.going_right;
+  zs = left ? V_ZL : V_ZR;
+  gs = left ? V_GL : V_GR;
 }
 
 // Chain index of the calling wavefront (`wpb` wavefronts per block), made
@@ -973,17 +1099,23 @@ persistent_kernel(M m, EngineCfg cfg, Collect col, const int64_t* __restrict__ d
     cx.lb = nax_slots_lds;
     cx.nl = nlds;
   }
-  float* zr = zreq + (size_t)c * cfg.Dp;
-  float* gi = gout + (size_t)c * cfg.Dp;
+  // the first call may take a gradient the host evaluated (initialisation);
+  // after that the potential reads the requested position from the chain's
+  // own edge slot and writes the gradient next to it, in LDS when the slots
+  // live there
+  const float* gi = gout + (size_t)c * cfg.Dp;
   const int wave_lane = threadIdx.x & (NAX_WAVE - 1);
   ChainState& S = sS[threadIdx.x / NAX_WAVE];
   copy_state(S, states[c], wave_lane);
   double pe = peout[c];
   int evals = 0;
-  while (advance_chain(cfg, col, target, S, cx, c, zr, gi, pe, evals)) {
+  while (advance_chain(cfg, col, target, S, cx, c, nullptr, gi, pe, evals)) {
+    int zs, gs;
+    request_slots(S, zs, gs);
     chain_sync(cx);
-    pe = m.value_and_grad(cx, zr, gi);
+    pe = m.value_and_grad(cx, cx.v(zs), cx.v(gs));
     chain_sync(cx);
+    gi = cx.v(gs);
   }
   if (nlds > 0) {
     __syncthreads();
