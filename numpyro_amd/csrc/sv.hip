@@ -51,6 +51,9 @@ __device__ __forceinline__ double digamma_pos(double x) {
 }
 
 constexpr double kLogPi = 1.1447298858494002;
+// Longest non-centred segment per thread kept in registers across the passes
+// (T = 3000 on 4 waves per chain: 12 steps per thread).
+constexpr int kSegRegs = 16;
 constexpr double kLog2Pi = 1.8378770664093453;
 
 struct SVModel {
@@ -62,9 +65,16 @@ struct SVModel {
   int softplus;       // positive sites by SoftplusTransform instead of ExpTransform
 
   // T-dependent constant part of U and of dU/dnu for the StudentT block.
+  // The two lgamma and two digamma values are evaluated on lanes 0 and 1 at
+  // once (same code, argument by lane), then read out of those lanes.
   __device__ void studentt_consts(double nu, double& u_const, double& dnu_const) const {
-    u_const = T * (0.5 * log(nu) + 0.5 * kLogPi + lgamma(0.5 * nu) - lgamma(0.5 * (nu + 1.0)));
-    dnu_const = T * (0.5 / nu + 0.5 * digamma_pos(0.5 * nu) - 0.5 * digamma_pos(0.5 * (nu + 1.0)));
+    const int ln = (int)(threadIdx.x & (NAX_WAVE - 1));
+    const double x = (ln & 1) ? 0.5 * (nu + 1.0) : 0.5 * nu;
+    const double lg = lgamma(x), dg = digamma_pos(x);
+    const double lg0 = read_lane(lg, 0), lg1 = read_lane(lg, 1);
+    const double dg0 = read_lane(dg, 0), dg1 = read_lane(dg, 1);
+    u_const = T * (0.5 * log(nu) + 0.5 * kLogPi + lg0 - lg1);
+    dnu_const = T * (0.5 / nu + 0.5 * dg0 - 0.5 * dg1);
   }
 
   template <class G>
@@ -132,6 +142,7 @@ struct SVModel {
     const int L = (T + G::NT - 1) / G::NT;
     const int t0 = lane * L;
     const int t1 = min(T, t0 + L);
+    if (L <= kSegRegs) return nc_short(grp, z, g, tn, ts, t0, t1);
     // pass 1: segment sums -> prefix checkpoints
     double seg = 0.0;
 #pragma unroll 8
@@ -174,6 +185,15 @@ struct SVModel {
       gz[t] = (float)((double)zt + sig * (after + A - P));
       P += (double)ds;
     }
+    return nc_tail(grp, g, tn, ts, U, Z2, asum, bsum, Asc);
+  }
+
+  // The group sums and the scalar terms of the non-centred form.
+  template <class G>
+  __device__ double nc_tail(const G& grp, float* g, const PosTf& tn, const PosTf& ts, double U,
+                            double Z2, double asum, double bsum, double Asc) const {
+    const double nu = tn.x, sig = ts.x;
+    const float nu1 = (float)(nu + 1.0);
     double v[5] = {U, Z2, asum, bsum, Asc};
     chain_sum_n(grp, v);
     U = v[0] + 0.5 * v[1];
@@ -184,12 +204,72 @@ struct SVModel {
     U += uc + T * 0.5 * kLog2Pi                         // N(0,1) normaliser of zeta
          - log(sigma_rate) + sigma_rate * sig - ts.ladj
          - log(nu_rate) + nu_rate * nu - tn.ladj;
-    if (lane == 0) {
+    if (grp.lane == 0) {
       g[0] = (float)(tn.dx * (dnu + dnc + nu_rate) - tn.dladj);
       g[1] = softplus ? (float)(ts.dx * (asc + sigma_rate) - ts.dladj)
                       : (float)(sig * asc + sigma_rate * sig - 1.0);
     }
     return U;
+  }
+
+  // The non-centred form for segments of at most kSegRegs steps: the three
+  // passes of nc() with the segment's zeta, returns and dU/ds kept in
+  // registers between them (the same expressions in the same order).
+  template <class G>
+  __device__ double nc_short(const G& grp, const float* z, float* g, const PosTf& tn,
+                             const PosTf& ts, int t0, int t1) const {
+    const double nu = tn.x, sig = ts.x;
+    const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
+    const float* zeta = z + 2;
+    float* gz = g + 2;
+    float zk[kSegRegs], rk[kSegRegs], dk[kSegRegs];
+#pragma unroll
+    for (int k = 0; k < kSegRegs; ++k) {
+      const int t = t0 + k;
+      zk[k] = t < t1 ? zeta[t] : 0.0f;
+      rk[k] = t < t1 ? r[t] : 0.0f;
+    }
+    double seg = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSegRegs; ++k)
+      if (t0 + k < t1) seg += (double)zk[k];
+    const double cp = chain_inclusive_scan(grp, seg) - seg;
+    const float inv_nu = 1.0f / nuf;
+    double U = 0.0, asum = 0.0, bsum = 0.0, A = 0.0, Asc = 0.0, Z2 = 0.0;
+    double c = cp;
+#pragma unroll
+    for (int k = 0; k < kSegRegs; ++k) {
+      dk[k] = 0.0f;
+      if (t0 + k < t1) {
+        const float zt = zk[k];
+        c += (double)zt;
+        const float s = (float)(sig * c);
+        const float rr = rk[k];
+        const float y2 = rr * rr * __expf(-2.0f * s);
+        const float a = log1pf(y2 * inv_nu);
+        const float b = y2 / (nuf + y2);
+        const float ds = 1.0f - nu1 * b;
+        dk[k] = ds;
+        U += 0.5 * (double)nu1 * (double)a + (double)s;
+        asum += (double)a;
+        bsum += (double)b;
+        A += (double)ds;
+        Asc += (double)ds * c;
+        Z2 += (double)zt * (double)zt;
+      }
+    }
+    const double Atot = chain_sum(grp, A);
+    const double after = Atot - chain_inclusive_scan(grp, A);  // sum over later segments
+    double P = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSegRegs; ++k) {
+      const int t = t0 + k;
+      if (t < t1) {
+        gz[t] = (float)((double)zk[k] + sig * (after + A - P));
+        P += (double)dk[k];
+      }
+    }
+    return nc_tail(grp, g, tn, ts, U, Z2, asum, bsum, Asc);
   }
 };
 
