@@ -65,10 +65,11 @@ __device__ __forceinline__ void setK(uint32_t* d, Key k) { d[0] = k.k0; d[1] = k
 #define NAX_FOR_D_UNROLL 1
 #endif
 
-// LDSV: the first nl slots may live in LDS (one-wavefront persistent
-// launches of long-vector models).  Only that context pays for the run-time
-// slot test and for generic (flat) vector pointers; the others address the
-// global vector block directly.
+// LDSV: the V_HOT leading slots live in LDS (persistent launches of
+// long-vector models), the others in the global vector block.  Slot numbers
+// are compile-time constants or provably below / above V_HOT wherever a pass
+// touches a vector, so the compiler addresses each one as LDS or as global
+// memory (no generic pointers).
 template <bool LDSV, int WPC_ = 1>
 struct ChainCtxT {
   static constexpr int WPC = WPC_;          // wavefronts working on the chain
@@ -79,8 +80,7 @@ struct ChainCtxT {
   double* red;     // LDS reduction slots of the group (kRedSlots * WPC), WPC > 1
   float* redf;     // LDS [WPC][64] fp32 exchange of the group, WPC > 1
   float* vb;       // chain's vector block [nvec][Dp]
-  float* lb;       // LDS copy of slots [0, nl) when LDSV, else unused
-  int nl;
+  float* lb;       // LDS copy of slots [0, V_HOT) when LDSV, else unused
   double* wel;     // Welford [2][Dp] (mean, m2) fp64
   int lane, D, Dp;  // lane: the thread's index in the chain's group, 0..NT-1
   // dense mass matrix ([Dp][Dp] row-major blocks, nullptr when diagonal):
@@ -89,7 +89,7 @@ struct ChainCtxT {
   float* ms;
   double* m2;
   __device__ float* v(int slot) const {
-    if constexpr (LDSV) return (slot < nl ? lb : vb) + (size_t)slot * Dp;
+    if constexpr (LDSV) return (slot < V_HOT ? lb : vb) + (size_t)slot * Dp;
     return vb + (size_t)slot * Dp;
   }
 };
@@ -973,7 +973,6 @@ __device__ __forceinline__ ChainCtxT<LDSV, WPC> make_ctx(const EngineCfg& cfg, i
   cx.Dp = cfg.Dp;
   cx.vb = vec + (size_t)c * cfg.nvec * cfg.Dp;
   cx.lb = nullptr;
-  cx.nl = 0;
   cx.wel = welf + (size_t)c * 3 * cfg.Dp;
   const size_t blk = (size_t)c * cfg.Dp * cfg.Dp;
   cx.mi = (DENSE && cfg.dense) ? cfg.dimm_t + blk : nullptr;
@@ -1048,27 +1047,26 @@ hipError_t launch_group_eval(int wpc, hipStream_t s, const M& m, const float* z,
   return hipGetLastError();
 }
 
-// Vector slots a persistent launch with one wavefront per workgroup keeps in
-// LDS: the leading slots (the per-leaf ones first) that fit in the CU's
-// 160 KB next to the kernel's static LDS, all of them when the chain's
-// vectors are short.  The host passes nl * Dp * 4 bytes of dynamic LDS.
-inline int persistent_lds_slots(int Dp, int nvec, int wpc = 1) {
+// Vector slots a persistent launch of one chain per workgroup keeps in LDS:
+// the V_HOT slots every leaf touches when they fit in the CU's 160 KB next to
+// the kernel's static LDS, else none.  The host passes nl * Dp * 4 bytes of
+// dynamic LDS.
+inline int persistent_lds_slots(int Dp, int wpc = 1) {
   // static LDS of the kernel: a ChainState copy per wave + the group's
   // reduction slots, and a margin
   const int budget = 160 * 1024 - 2048 -
                      wpc * (int)(sizeof(ChainState) + kRedSlots * 8 + NAX_WAVE * 4);
-  const int fit = budget / (Dp * 4);
-  return fit >= nvec ? nvec : (fit >= V_HOT ? V_HOT : fit);
+  return budget / (Dp * 4) >= V_HOT ? V_HOT : 0;
 }
 
 // The whole sampling loop of one chain in one launch: the wave alternates
 // advance_chain() and its own potential until the chain reaches `target`.
 // WPB wavefronts (chains) per workgroup: one, for latency-bound chains with
 // long vectors, spreads the chains over as many CUs (and their L1 caches and
-// address units) as there are chains.  LDSV (with WPB = 1): the first nl
-// vector slots live in LDS for the launch (copied in and back out), so a
-// leaf's vector passes run at LDS rather than L2 latency; for models with
-// long latent vectors.
+// address units) as there are chains.  LDSV (with WPB = 1): the nl = V_HOT
+// leading vector slots live in LDS for the launch (copied in and back out),
+// so a leaf's vector passes run at LDS rather than L2 latency; for models
+// with long latent vectors.
 // WPC > 1 (with WPB = 1): WPC wavefronts of the workgroup form the chain's
 // group; each wave keeps its own copy of the chain state, wave 0 writes it
 // back.
@@ -1091,13 +1089,12 @@ persistent_kernel(M m, EngineCfg cfg, Collect col, const int64_t* __restrict__ d
   ChainCtxT<LDSV, WPC> cx = make_ctx<DENSE, LDSV, WPC>(cfg, c, vec, welf);
   cx.red = nax_red;
   cx.redf = nax_redf;
-  const int nlds = (LDSV && WPB == 1) ? nl : 0;
-  if (nlds > 0) {
-    const int n = nlds * cfg.Dp;
+  static_assert(!LDSV || WPB == 1, "LDS-resident slots need one chain per workgroup");
+  if constexpr (LDSV) {
+    const int n = nl * cfg.Dp;
     for (int i = cx.lane; i < n; i += cx.NT) nax_slots_lds[i] = cx.vb[i];
     __syncthreads();
     cx.lb = nax_slots_lds;
-    cx.nl = nlds;
   }
   // the first call may take a gradient the host evaluated (initialisation);
   // after that the potential reads the requested position from the chain's
@@ -1117,9 +1114,9 @@ persistent_kernel(M m, EngineCfg cfg, Collect col, const int64_t* __restrict__ d
     chain_sync(cx);
     gi = cx.v(gs);
   }
-  if (nlds > 0) {
+  if constexpr (LDSV) {
     __syncthreads();
-    const int n = nlds * cfg.Dp;
+    const int n = nl * cfg.Dp;
     for (int i = cx.lane; i < n; i += cx.NT) cx.vb[i] = nax_slots_lds[i];
   }
   if (cx.lane < NAX_WAVE) copy_state(states[c], S, wave_lane);
@@ -1164,7 +1161,8 @@ static __global__ void step_target_kernel(int64_t* t, int64_t final, int64_t ste
 static __global__ void set_target_kernel(int64_t* t, int64_t v) { *t = v; }
 
 // Launches persistent_kernel<M, ., 1, LDSV, wpc> (one chain per workgroup),
-// with the vector slots in LDS when LDSV.  Dense mass runs on one wave.
+// with the hot vector slots in LDS when LDSV and they fit (else the global
+// form).  Dense mass runs on one wave.
 template <class M, bool LDSV>
 hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineCfg& cfg,
                                const Collect& col, const int64_t* d_target, ChainState* st,
@@ -1173,19 +1171,27 @@ hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineC
   if (cfg.dense) wpc = 1;
 #define NAX_GROUP_LAUNCH(DN, W)                                                             \
   {                                                                                         \
-    const int nl = LDSV ? persistent_lds_slots(cfg.Dp, cfg.nvec, W) : 0;                    \
-    const size_t lds = (size_t)nl * cfg.Dp * sizeof(float);                                 \
-    static size_t lds_set = 0; /* set once per size (also outside graph capture) */        \
-    if (LDSV && lds > lds_set) {                                                            \
-      hipError_t e = hipFuncSetAttribute(                                                   \
-          (const void*)persistent_kernel<M, DN, 1, LDSV, W>,                                \
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                            \
-      if (e != hipSuccess) return e;                                                        \
-      lds_set = lds;                                                                        \
+    const int nl = LDSV ? persistent_lds_slots(cfg.Dp, W) : 0;                              \
+    if constexpr (LDSV) {                                                                   \
+      if (nl > 0) {                                                                         \
+        const size_t lds = (size_t)nl * cfg.Dp * sizeof(float);                             \
+        static size_t lds_set = 0; /* set once per size (also outside graph capture) */    \
+        if (lds > lds_set) {                                                                \
+          hipError_t e = hipFuncSetAttribute(                                               \
+              (const void*)persistent_kernel<M, DN, 1, true, W>,                            \
+              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                        \
+          if (e != hipSuccess) return e;                                                    \
+          lds_set = lds;                                                                    \
+        }                                                                                   \
+        hipLaunchKernelGGL((persistent_kernel<M, DN, 1, true, W>), dim3(cfg.C),             \
+                           dim3(64 * W), lds, s, m, cfg, col, d_target, st, vec, wel, zreq, \
+                           gout, peout, gevals, nl);                                        \
+        return hipGetLastError();                                                           \
+      }                                                                                     \
     }                                                                                       \
-    hipLaunchKernelGGL((persistent_kernel<M, DN, 1, LDSV, W>), dim3(cfg.C), dim3(64 * W),   \
-                       lds, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout,      \
-                       gevals, nl);                                                         \
+    hipLaunchKernelGGL((persistent_kernel<M, DN, 1, false, W>), dim3(cfg.C), dim3(64 * W), \
+                       0, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout,        \
+                       gevals, 0);                                                          \
     return hipGetLastError();                                                               \
   }
   if (cfg.dense) NAX_GROUP_LAUNCH(true, 1)
