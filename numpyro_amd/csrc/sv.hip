@@ -78,12 +78,12 @@ struct SVModel {
   }
 
   template <class G>
-  __device__ double value_and_grad(const G& grp, const float* z, float* g) const {
+  __device__ __forceinline__ double value_and_grad(const G& grp, const float* z, float* g) const {
     return noncentred ? nc(grp, z, g) : centred(grp, z, g);
   }
 
   template <class G>
-  __device__ double centred(const G& grp, const float* z, float* g) const {
+  __device__ __forceinline__ double centred(const G& grp, const float* z, float* g) const {
     const int lane = grp.lane;
     const double u_nu = z[0], u_sig = z[T + 1];
     const PosTf tn = positive_tf(u_nu, softplus), ts = positive_tf(u_sig, softplus);
@@ -131,7 +131,7 @@ struct SVModel {
   }
 
   template <class G>
-  __device__ double nc(const G& grp, const float* z, float* g) const {
+  __device__ __forceinline__ double nc(const G& grp, const float* z, float* g) const {
     const int lane = grp.lane;
     const double u_nu = z[0], u_sig = z[1];
     const PosTf tn = positive_tf(u_nu, softplus), ts = positive_tf(u_sig, softplus);
@@ -190,8 +190,9 @@ struct SVModel {
 
   // The group sums and the scalar terms of the non-centred form.
   template <class G>
-  __device__ double nc_tail(const G& grp, float* g, const PosTf& tn, const PosTf& ts, double U,
-                            double Z2, double asum, double bsum, double Asc) const {
+  __device__ __forceinline__ double nc_tail(const G& grp, float* g, const PosTf& tn,
+                                            const PosTf& ts, double U, double Z2, double asum,
+                                            double bsum, double Asc) const {
     const double nu = tn.x, sig = ts.x;
     const float nu1 = (float)(nu + 1.0);
     double v[5] = {U, Z2, asum, bsum, Asc};
@@ -216,8 +217,9 @@ struct SVModel {
   // passes of nc() with the segment's zeta, returns and dU/ds kept in
   // registers between them (the same expressions in the same order).
   template <class G>
-  __device__ double nc_short(const G& grp, const float* z, float* g, const PosTf& tn,
-                             const PosTf& ts, int t0, int t1) const {
+  __device__ __forceinline__ double nc_short(const G& grp, const float* z, float* g,
+                                             const PosTf& tn, const PosTf& ts, int t0,
+                                             int t1) const {
     const double nu = tn.x, sig = ts.x;
     const float nuf = (float)nu, nu1 = (float)(nu + 1.0);
     const float* zeta = z + 2;
