@@ -60,6 +60,25 @@ def _run_oracle(X, y, z0, keys, num_warmup, num_samples, algo="NUTS", **kw):
     return [np.stack(res[k]) for k in ("z", "num_steps", "potential_energy", "accept_prob")]
 
 
+@pytest.mark.parametrize("kernel", ["tile", "chain"])
+def test_deep_trees_match_oracle(kernel):
+    # a small fixed step: trees of 32-1023 leaves, so a leaf's U-turn checks
+    # run against up to depth checkpoints (three in the leaf's fused pass,
+    # the rest one by one); tick kernel ("tile") and persistent chain group
+    N, D, C = 200, 5, 3
+    X, y = _problem(N, D, 31)
+    rng = np.random.default_rng(131)
+    z0 = rng.uniform(-1, 1, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(31), C)[c] for c in range(C)])
+    kw = dict(adapt_step_size=False, adapt_mass_matrix=False, step_size=0.01)
+    gz, gns, gpe, _ = _run_engine(X, y, z0, keys, 0, 12, model_kw={"kernel": kernel}, **kw)
+    oz, ons, ope, _ = _run_oracle(X, y, z0, keys, 0, 12, **kw)
+    assert ons.max() >= 31, ons
+    assert np.array_equal(gns, ons), (gns, ons)
+    np.testing.assert_allclose(gz, oz, rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(gpe, ope, rtol=1e-5)
+
+
 @pytest.mark.parametrize("seed, adapt, regularize", [(11, True, True), (12, False, True),
                                                      (13, True, False)])
 def test_nuts_chain_matches_oracle(seed, adapt, regularize):
