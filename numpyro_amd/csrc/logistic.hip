@@ -499,7 +499,7 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
   const bool fp8 = x_dtype == NAX_X_FP8;
   if (!fp8 && (fuse_variant == 3 || (fuse_variant == 0 && num_chains <= 16 && d <= 64 &&
                                      (double)n * d <= (double)(1 << 20))))
-    return make_logistic_chain(x, y, n, d, prior_scale, err);
+    return make_logistic_chain(x, y, n, d, prior_scale, s, err);
   const bool mfma = !fp8 && (fuse_variant == 2 ||
                              (fuse_variant == 0 && num_chains >= 32 && d <= 256));
   if (mfma && !(fuse_variant == 0 && num_chains <= 128 && d <= 320))

@@ -10,7 +10,9 @@
 //
 // Lane = row: each lane takes rows lane, lane + 64, ... and keeps the
 // chain's position (d <= DMAX) and its own partial X^T r in registers; one
-// wave reduction per coordinate finishes the gradient.
+// wave reduction per coordinate finishes the gradient.  The model reads a
+// column-major copy of X made at set-up, so that the lanes' reads of one
+// coordinate of consecutive rows are contiguous.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -24,8 +26,8 @@ namespace nax {
 
 template <int DMAX>
 struct LogisticChainModel {
-  const float* x;  // [n, d] row-major fp32
-  const float* y;  // [n]
+  const float* xt;  // [d, n] column-major copy of X, fp32
+  const float* y;   // [n]
   int64_t n;
   int d;
   float inv_s2;
@@ -42,12 +44,11 @@ struct LogisticChainModel {
     }
     double u = 0.0;
     for (int64_t row = grp.lane; row < n; row += G::NT) {
-      const float* xr = x + row * d;
       float xv[DMAX];
       float l0 = 0.0f, l1 = 0.0f;
 #pragma unroll
       for (int k = 0; k < DMAX; ++k) {
-        xv[k] = k < d ? xr[k] : 0.0f;
+        xv[k] = k < d ? xt[(int64_t)k * n + row] : 0.0f;
         if (k & 1) l1 = fmaf(xv[k], zr[k], l1);
         else l0 = fmaf(xv[k], zr[k], l0);
       }
@@ -108,6 +109,8 @@ struct LogisticChainModel {
 template <int DMAX>
 struct LogisticChainPotential : Potential {
   LogisticChainModel<DMAX> m;
+  float* xt = nullptr;  // owned column-major copy of X
+  ~LogisticChainPotential() override { (void)hipFree(xt); }
   double flops = 0.0, bytes = 0.0;
   int dim() const override { return m.d; }
   const char* name() const override { return "logistic_chain"; }
@@ -170,15 +173,37 @@ struct LogisticChainPotential : Potential {
   bool forced = false;
 };
 
-// d <= 64; x and y are the caller's device arrays (kept, not copied).
+// xt[k n + i] = x[i d + k]
+__global__ void transpose_rows_kernel(const float* __restrict__ x, int64_t n, int d,
+                                      float* __restrict__ xt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * d) return;
+  xt[(i % d) * n + i / d] = x[i];
+}
+
+// d <= 64; y is the caller's device array (kept, not copied), X is copied
+// column-major.
 Potential* make_logistic_chain(const float* x, const float* y, int64_t n, int d,
-                               double prior_scale, std::string* err) {
+                               double prior_scale, hipStream_t s, std::string* err) {
   if (d < 1 || d > 64) {
     *err = "the per-chain logistic kernel supports 1 <= dim <= 64";
     return nullptr;
   }
-  auto fill = [&](auto* p) {
-    p->m.x = x;
+  auto fill = [&](auto* p) -> Potential* {
+    const int64_t nd = n * d;
+    hipError_t e = hipMalloc(&p->xt, sizeof(float) * (size_t)(nd > 0 ? nd : 1));
+    if (e == hipSuccess && nd > 0) {
+      hipLaunchKernelGGL(transpose_rows_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0,
+                         s, x, n, d, p->xt);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      *err = std::string("per-chain logistic model: ") + hipGetErrorString(e);
+      delete p;
+      return nullptr;
+    }
+    p->m.xt = p->xt;
     p->m.y = y;
     p->m.n = n;
     p->m.d = d;

@@ -152,7 +152,7 @@ Potential* make_logistic(const float* x, const float* y, int64_t n, int d, doubl
                          std::string* err);
 
 Potential* make_logistic_chain(const float* x, const float* y, int64_t n, int d,
-                               double prior_scale, std::string* err);
+                               double prior_scale, hipStream_t s, std::string* err);
 
 Potential* make_hier(const float* x, const float* y, const int64_t* row_off, const int64_t* elt_off,
                      const int* dims, int groups, int chains_per_group, int num_chains, int Dp,
