@@ -129,8 +129,10 @@ struct LogisticChainPotential : Potential {
   hipError_t prepare_persistent(hipStream_t s, const EngineCfg& cfg, ChainState* st, float* vec,
                                 double* wel, float* zreq, float* gout, double* peout) override {
     // one chain per workgroup; the waves per chain (rows of X split over
-    // 64 wpc threads) are picked on the first launch by timing a few
-    // iterations of each candidate from a scratch copy of the state
+    // 64 wpc threads; the chain's vectors fit one wave, so with more than one
+    // wave the first runs the state machine alone) are picked on the first
+    // launch by timing a few iterations of each candidate from a scratch
+    // copy of the state
     if (!forced && (wpc == 0 || tuned_chains != cfg.C)) {
       const std::string key = "logistic_chain n=" + std::to_string(m.n) + " d=" +
                               std::to_string(m.d) + " C=" + std::to_string(cfg.C) +
@@ -142,8 +144,8 @@ struct LogisticChainPotential : Potential {
         return hipSuccess;
       }
       std::string log;
-      wpc = tune_chain_groups<LogisticChainModel<DMAX>, false>(s, m, cfg, st, vec, wel, zreq,
-                                                               gout, peout, 4, &log);
+      wpc = tune_chain_groups<LogisticChainModel<DMAX>, false, true>(s, m, cfg, st, vec, wel,
+                                                                     zreq, gout, peout, 4, &log);
       tuned_shape(key, &wpc, true);
       tuning = "logistic_chain: " + std::to_string(wpc) + " waves per chain (ms for 4 "
                "iterations:" + log + ")";
@@ -154,7 +156,7 @@ struct LogisticChainPotential : Potential {
                             const int64_t* d_target, ChainState* st, float* vec, double* wel,
                             float* zreq, float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    hipError_t e = launch_chain_groups<LogisticChainModel<DMAX>, false>(
+    hipError_t e = launch_chain_groups<LogisticChainModel<DMAX>, false, true>(
         wpc > 0 ? wpc : 1, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals);
     mark_end(s);
     return e;

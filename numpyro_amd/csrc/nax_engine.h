@@ -1127,6 +1127,76 @@ persistent_kernel(M m, EngineCfg cfg, Collect col, const int64_t* __restrict__ d
 }
 
 
+// The same loop for chains whose vectors fit one wavefront (d <= 64) on a
+// group of WPC waves: the group's wave 0 alone runs advance_chain() (vector
+// passes of one element per lane and wave-level sums, no workgroup barrier),
+// the whole group evaluates the potential (rows split over its 64 WPC
+// threads).  The head publishes whether it needs a gradient and the request's
+// slots in LDS; every wave leaves the loop after the same barrier.
+template <class M, bool LDSV, int WPC>
+__global__ void __launch_bounds__(64 * WPC)
+persistent_head_kernel(M m, EngineCfg cfg, Collect col, const int64_t* __restrict__ d_target,
+                       ChainState* __restrict__ states, float* __restrict__ vec,
+                       double* __restrict__ welf, float* __restrict__ zreq,
+                       float* __restrict__ gout, double* __restrict__ peout,
+                       int64_t* __restrict__ grad_evals, int nl) {
+  static_assert(WPC > 1, "the head wave form is for chain groups");
+  const int64_t target = *d_target;
+  __shared__ ChainState sS;
+  __shared__ double nax_red[kRedSlots * WPC];
+  __shared__ float nax_redf[NAX_WAVE * WPC];
+  __shared__ int req[3];  // needs a gradient, position slot, gradient slot
+  extern __shared__ float nax_slots_lds[];
+  const int c = (int)blockIdx.x;
+  if (c >= cfg.C) return;
+  const bool head = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == 0;
+  ChainCtxT<LDSV, WPC> gx = make_ctx<false, LDSV, WPC>(cfg, c, vec, welf);
+  gx.red = nax_red;
+  gx.redf = nax_redf;
+  ChainCtxT<LDSV, 1> hx = make_ctx<false, LDSV, 1>(cfg, c, vec, welf);
+  if constexpr (LDSV) {
+    const int n = nl * cfg.Dp;
+    for (int i = gx.lane; i < n; i += gx.NT) nax_slots_lds[i] = gx.vb[i];
+    __syncthreads();
+    gx.lb = nax_slots_lds;
+    hx.lb = nax_slots_lds;
+  }
+  // the first call may take a gradient the host evaluated (initialisation)
+  const float* gi = gout + (size_t)c * cfg.Dp;
+  if (head) copy_state(sS, states[c], hx.lane);
+  double pe = peout[c];
+  int evals = 0;
+  for (;;) {
+    if (head) {
+      const bool more = advance_chain(cfg, col, target, sS, hx, c, nullptr, gi, pe, evals);
+      if (hx.lane == 0) {
+        int zs, gs;
+        request_slots(sS, zs, gs);
+        req[0] = more ? 1 : 0;
+        req[1] = zs;
+        req[2] = gs;
+      }
+    }
+    __syncthreads();
+    const int more = req[0], zs = req[1], gs = req[2];
+    if (!more) break;
+    pe = m.value_and_grad(gx, gx.v(zs), gx.v(gs));
+    __syncthreads();  // the gradient is written, req has been read
+    gi = gx.v(gs);
+  }
+  if constexpr (LDSV) {
+    const int n = nl * cfg.Dp;
+    for (int i = gx.lane; i < n; i += gx.NT) gx.vb[i] = nax_slots_lds[i];
+  }
+  if (head) {
+    copy_state(states[c], sS, hx.lane);
+    if (hx.lane == 0) {
+      peout[c] = pe;
+      if (evals && grad_evals) atomicAdd((unsigned long long*)grad_evals, (unsigned long long)evals);
+    }
+  }
+}
+
 }  // namespace nax
 
 // ---- host: persistent launches with a run-time number of waves per chain ---------
@@ -1162,8 +1232,9 @@ static __global__ void set_target_kernel(int64_t* t, int64_t v) { *t = v; }
 
 // Launches persistent_kernel<M, ., 1, LDSV, wpc> (one chain per workgroup),
 // with the hot vector slots in LDS when LDSV and they fit (else the global
-// form).  Dense mass runs on one wave.
-template <class M, bool LDSV>
+// form).  Dense mass runs on one wave.  HEAD (models whose vectors fit one
+// wave): groups of more than one wave run persistent_head_kernel instead.
+template <class M, bool LDSV, bool HEAD = false>
 hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineCfg& cfg,
                                const Collect& col, const int64_t* d_target, ChainState* st,
                                float* vec, double* wel, float* zreq, float* gout,
@@ -1171,6 +1242,12 @@ hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineC
   if (cfg.dense) wpc = 1;
 #define NAX_GROUP_LAUNCH(DN, W)                                                             \
   {                                                                                         \
+    if constexpr (HEAD && !LDSV && W > 1 && !DN) {                                          \
+      hipLaunchKernelGGL((persistent_head_kernel<M, false, W>), dim3(cfg.C), dim3(64 * W),  \
+                         0, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout,      \
+                         gevals, 0);                                                        \
+      return hipGetLastError();                                                             \
+    }                                                                                       \
     const int nl = LDSV ? persistent_lds_slots(cfg.Dp, W) : 0;                              \
     if constexpr (LDSV) {                                                                   \
       if (nl > 0) {                                                                         \
@@ -1275,7 +1352,7 @@ int tune_persistent(hipStream_t s, const EngineCfg& cfg, const ChainState* st, c
 
 // Waves per chain of a persistent per-chain model (kGroupSizes, one wave
 // under dense mass), by tune_persistent.
-template <class M, bool LDSV>
+template <class M, bool LDSV, bool HEAD = false>
 int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const ChainState* st,
                       const float* vec, const double* wel, const float* zreq, const float* gout,
                       const double* peout, int iters, std::string* log) {
@@ -1284,8 +1361,8 @@ int tune_chain_groups(hipStream_t s, const M& m, const EngineCfg& cfg, const Cha
   auto launch = [&](int w, hipStream_t ss, const Collect& col, const int64_t* tgt,
                     ChainState* st2, float* vec2, double* wel2, float* zr2, float* g2,
                     double* pe2, int64_t* ge2) {
-    return launch_chain_groups<M, LDSV>(w, ss, m, cfg, col, tgt, st2, vec2, wel2, zr2, g2, pe2,
-                                        ge2);
+    return launch_chain_groups<M, LDSV, HEAD>(w, ss, m, cfg, col, tgt, st2, vec2, wel2, zr2, g2,
+                                              pe2, ge2);
   };
   return tune_persistent(s, cfg, st, vec, wel, zreq, gout, peout, iters, cands, launch, log);
 }
