@@ -816,12 +816,14 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
           S.s_peP = pe_new; S.s_energyP = e_new; S.s_weight = leaf_w;
           S.s_diverging = leaf_div; S.s_sum_accept = acc; S.s_num = 1;
         } else {  // _combine_tree(..., biased_transition=False) (hmc_util.py:767-848)
-          const double p = expit(leaf_w - S.s_weight);
+          const double x = leaf_w - S.s_weight;
+          const double e = exp(-fabs(x));
+          const double p = expit_e(x, e);
           if (bernoulli(K(S.leaf_key), p, part)) {
             FOR_D(d) { szp[d] = znew[d]; sgp[d] = gnew[d]; }
             S.s_peP = pe_new; S.s_energyP = e_new;
           }
-          S.s_weight = log_add_exp(S.s_weight, leaf_w);
+          S.s_weight = log_add_exp_e(S.s_weight, leaf_w, e);
           S.s_diverging = leaf_div;
           S.s_sum_accept += acc;
           S.s_num += 1;
@@ -835,8 +837,12 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
         float* rsum = cx.v(V_RSUM);
         const float* srs = cx.v(V_SRSUM);
         FOR_D(d) rsum[d] += srs[d];
+        // min(1, exp(x)) and log_add_exp's exp(-|x|) from one exp: for x <= 0
+        // they are the same number, for x > 0 (or NaN) the probability is 1
+        const double x = S.s_weight - S.weight;
+        const double e = exp(-fabs(x));
         double p = 0.0;
-        if (!S.s_turning && !S.s_diverging) p = fmin(exp(fmin(S.s_weight - S.weight, 700.0)), 1.0);
+        if (!S.s_turning && !S.s_diverging) p = x <= 0.0 ? e : 1.0;
         const bool turn = S.s_turning || is_turning(cx, cx.v(V_RL), cx.v(V_RR), rsum, nullptr, nullptr);
         if (bernoulli(K(S.top_key), p, part)) {
           vcopy(cx, V_ZP, V_SZP);
@@ -845,7 +851,7 @@ __device__ __forceinline__ bool advance_chain(const EngineCfg& cfg, const Collec
           S.energyP = S.s_energyP;
         }
         S.depth += 1;
-        S.weight = log_add_exp(S.weight, S.s_weight);
+        S.weight = log_add_exp_e(S.weight, S.s_weight, e);
         S.turning = turn ? 1 : 0;
         S.t_diverging = S.s_diverging;
         S.sum_accept += S.s_sum_accept;
