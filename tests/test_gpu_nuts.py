@@ -297,3 +297,32 @@ def test_chain_kernel_adapted_matches_oracle():
         bad = np.flatnonzero(gns[c] != ons[c])
         assert bad.size == 0 or bad[0] >= 25, (c, gns[c], ons[c])
     np.testing.assert_allclose(gz[:, :15], oz[:, :15], rtol=1e-3, atol=1e-3)
+
+
+@pytest.mark.parametrize("waves", [1, 2, 8, 16])
+def test_chain_kernel_fixed_waves_match_oracle(waves):
+    # the per-chain form at a fixed number of waves per chain: one wave, or a
+    # group whose first wave runs the state machine alone while the whole
+    # group evaluates the potential; graph replay and direct launches give
+    # the same draws, and the draws follow the oracle
+    from numpyro_amd.infer.hmc import Engine
+    from numpyro_amd.models import LogisticRegression
+    N, D, C = 300, 6, 4
+    X, y = _problem(N, D, 63)
+    z0 = np.random.default_rng(16).uniform(-2, 2, size=(C, D)).astype(np.float32)
+    keys = np.stack([ojr.split(ojr.PRNGKey(43), C)[c] for c in range(C)])
+    kw = dict(adapt_step_size=False, adapt_mass_matrix=False, step_size=0.2)
+    outs = []
+    for graphs in (True, False):
+        e = Engine(LogisticRegression(X, y, kernel="chain"), C, "NUTS", 0, **kw)
+        e.set_graphs(graphs)
+        e.set_launch_shape(waves)
+        e.init(torch.from_numpy(z0), keys)
+        out = e.run(40, collect_fields=("z", "num_steps"), size=40)
+        outs.append({k: v.cpu().numpy() for k, v in out.items()})
+        assert f"{waves} waves per chain" in e.tuning(), e.tuning()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    oz, ons, _, _ = _run_oracle(X, y, z0, keys, 0, 40, **kw)
+    assert np.array_equal(outs[0]["num_steps"], ons), (outs[0]["num_steps"], ons)
+    np.testing.assert_allclose(outs[0]["z"], oz, rtol=1e-3, atol=1e-3)
