@@ -144,7 +144,7 @@ struct LogisticChainPotential : Potential {
         return hipSuccess;
       }
       std::string log;
-      wpc = tune_chain_groups<LogisticChainModel<DMAX>, false, true>(s, m, cfg, st, vec, wel,
+      wpc = tune_chain_groups<LogisticChainModel<DMAX>, true, true>(s, m, cfg, st, vec, wel,
                                                                      zreq, gout, peout, 4, &log);
       tuned_shape(key, &wpc, true);
       tuning = "logistic_chain: " + std::to_string(wpc) + " waves per chain (ms for 4 "
@@ -156,7 +156,7 @@ struct LogisticChainPotential : Potential {
                             const int64_t* d_target, ChainState* st, float* vec, double* wel,
                             float* zreq, float* gout, double* peout, int64_t* gevals) override {
     mark_begin(s);
-    hipError_t e = launch_chain_groups<LogisticChainModel<DMAX>, false, true>(
+    hipError_t e = launch_chain_groups<LogisticChainModel<DMAX>, true, true>(
         wpc > 0 ? wpc : 1, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout, gevals);
     mark_end(s);
     return e;

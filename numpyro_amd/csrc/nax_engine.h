@@ -1248,7 +1248,25 @@ hipError_t launch_chain_groups(int wpc, hipStream_t s, const M& m, const EngineC
   if (cfg.dense) wpc = 1;
 #define NAX_GROUP_LAUNCH(DN, W)                                                             \
   {                                                                                         \
-    if constexpr (HEAD && !LDSV && W > 1 && !DN) {                                          \
+    if constexpr (HEAD && W > 1 && !DN) {                                                   \
+      const int nlh = LDSV ? persistent_lds_slots(cfg.Dp, W) : 0;                           \
+      if constexpr (LDSV) {                                                                 \
+        if (nlh > 0) {                                                                      \
+          const size_t lds = (size_t)nlh * cfg.Dp * sizeof(float);                          \
+          static size_t lds_set = 0; /* set once per size (also outside graph capture) */  \
+          if (lds > lds_set) {                                                              \
+            hipError_t e = hipFuncSetAttribute(                                             \
+                (const void*)persistent_head_kernel<M, true, W>,                            \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                      \
+            if (e != hipSuccess) return e;                                                  \
+            lds_set = lds;                                                                  \
+          }                                                                                 \
+          hipLaunchKernelGGL((persistent_head_kernel<M, true, W>), dim3(cfg.C),             \
+                             dim3(64 * W), lds, s, m, cfg, col, d_target, st, vec, wel,     \
+                             zreq, gout, peout, gevals, nlh);                               \
+          return hipGetLastError();                                                         \
+        }                                                                                   \
+      }                                                                                     \
       hipLaunchKernelGGL((persistent_head_kernel<M, false, W>), dim3(cfg.C), dim3(64 * W),  \
                          0, s, m, cfg, col, d_target, st, vec, wel, zreq, gout, peout,      \
                          gevals, 0);                                                        \
